@@ -1,0 +1,85 @@
+// Device building blocks for the gfx950 (CDNA4) vector-sparse kernels.
+//
+// Wave64 MFMA operand conventions used throughout (v_mfma_i32_16x16x64_i8):
+//   lane l = r + 16*g   (r = l % 16, g = l / 16)
+//   A operand: 16 int8 of row r, reduction slots [16g, 16g+16)
+//   B operand: 16 int8 of column r, the same reduction slots
+//   C/D:       4 int32 per lane, rows 4g+i (i = 0..3) of column r
+// Only the pairing "A byte b of lane group g <-> B byte b of lane group g"
+// matters for the sum, so every loader below places the byte for reduction
+// slot s of its group at the same byte position in A and in B.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mc {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v2i __attribute__((ext_vector_type(2)));
+typedef unsigned int u32;
+
+__device__ __forceinline__ v4i mfma_i8(v4i a, v4i b, v4i c) {
+    return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ u32 perm(u32 hi, u32 lo, u32 sel) {
+    // byte selector over the 8-byte value {hi:lo}; 0..3 pick lo, 4..7 pick hi
+    return __builtin_amdgcn_perm(hi, lo, sel);
+}
+
+// 4x4 byte transpose: r[k] holds 4 bytes (columns 0..3) of row k; o[c] gets
+// column c as bytes (row0, row1, row2, row3).
+__device__ __forceinline__ void transpose4x4(u32 r0, u32 r1, u32 r2, u32 r3, u32 o[4]) {
+    const u32 t01l = perm(r1, r0, 0x05010400u);   // r0b0 r1b0 r0b1 r1b1
+    const u32 t01h = perm(r1, r0, 0x07030602u);   // r0b2 r1b2 r0b3 r1b3
+    const u32 t23l = perm(r3, r2, 0x05010400u);
+    const u32 t23h = perm(r3, r2, 0x07030602u);
+    o[0] = perm(t23l, t01l, 0x05040100u);
+    o[1] = perm(t23l, t01l, 0x07060302u);
+    o[2] = perm(t23h, t01h, 0x05040100u);
+    o[3] = perm(t23h, t01h, 0x07060302u);
+}
+
+// Nibble helpers on packed dwords (8 nibbles, nibble 0 in bits 0..3).
+__device__ __forceinline__ u32 nib_even(u32 x) { return x & 0x0F0F0F0Fu; }          // nibbles 0,2,4,6
+__device__ __forceinline__ u32 nib_odd(u32 x) { return (x >> 4) & 0x0F0F0F0Fu; }    // nibbles 1,3,5,7
+// Sign-extend four 4-bit values held in the low nibble of each byte.
+__device__ __forceinline__ u32 sext4_bytes(u32 t) {
+    const u32 m = t & 0x08080808u;
+    return t | ((m << 5) - (m << 1));   // 30*m == 0xF0 per flagged byte, no carries
+}
+
+__device__ __forceinline__ int sum_bytes(v4i a) {
+    int s = __builtin_amdgcn_sdot4(a[0], 0x01010101, 0, false);
+    s = __builtin_amdgcn_sdot4(a[1], 0x01010101, s, false);
+    s = __builtin_amdgcn_sdot4(a[2], 0x01010101, s, false);
+    s = __builtin_amdgcn_sdot4(a[3], 0x01010101, s, false);
+    return s;
+}
+
+// Raw buffer resource over [base, base + bytes): loads whose offset falls
+// outside return zero and touch no memory (gfx9 descriptor word 3).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, u32 bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ u32 buf_load_b32(__amdgpu_buffer_rsrc_t r, u32 off) {
+    return (u32)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+__device__ __forceinline__ v2i buf_load_b64(__amdgpu_buffer_rsrc_t r, u32 off) {
+    return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+}
+__device__ __forceinline__ v4i buf_load_b128(__amdgpu_buffer_rsrc_t r, u32 off) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+}
+
+// Cross-lane read of lane (self + delta) within the wave (delta multiple of 16).
+__device__ __forceinline__ int lane_down(int x, int delta) {
+    const int lane = __lane_id();
+    return __builtin_amdgcn_ds_bpermute(((lane + delta) & 63) << 2, x);
+}
+__device__ __forceinline__ int lane_read(int x, int src_lane) {
+    return __builtin_amdgcn_ds_bpermute((src_lane & 63) << 2, x);
+}
+
+}  // namespace mc

@@ -1,0 +1,133 @@
+// C ABI of the SpMM family: argument validation, tile-config table and
+// dispatch to the per-pair instantiation units.
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "spmm_dispatch.hpp"
+
+namespace mc {
+namespace {
+
+std::mutex g_cfg_mutex;
+std::map<std::tuple<int, int, int, int>, int> g_cfg_table;  // (lhs, rhs, V, n) -> config
+
+int default_config(int lhs_bits, int rhs_bits, int V, int n) {
+    {
+        std::lock_guard<std::mutex> lk(g_cfg_mutex);
+        auto it = g_cfg_table.find(std::make_tuple(lhs_bits, rhs_bits, V, n));
+        if (it != g_cfg_table.end()) return it->second;
+    }
+    (void)lhs_bits; (void)V;
+    const int cpd = 32 / rhs_bits;
+    const int n_dw = n / cpd;
+    // Wider lanes when the row allows it; a single config otherwise.
+    if (rhs_bits == 8 && n_dw % 2 == 0 && n >= 256) return 1;
+    return 0;
+}
+
+bool pair_supported(int lhs_bits, int rhs_bits) {
+    return (lhs_bits == 8 && rhs_bits == 8) || (lhs_bits == 16 && rhs_bits == 8) ||
+           (lhs_bits == 8 && rhs_bits == 4) || (lhs_bits == 4 && rhs_bits == 4);
+}
+
+mc_status spmm_entry(int lhs_bits, int rhs_bits, mc_signedness sign, int config, int m_vec, int vec_length, int n,
+                     int k, const int *row_indices, const int *row_offsets, const int *column_indices,
+                     const int *values, const int *rhs_matrix, int *output_matrix, hipStream_t stream) {
+    if (!pair_supported(lhs_bits, rhs_bits)) return MC_EUNSUPPORTED;
+    if (sign != MC_SIGNED && sign != MC_UNSIGNED) return MC_EINVAL_ARG;
+    if (!valid_v(vec_length)) return MC_EINVAL_V;
+    if (m_vec < 0 || n <= 0 || k <= 0) return MC_EINVAL_SHAPE;
+    const int cpd = 32 / rhs_bits;
+    if (n % cpd != 0) return MC_EINVAL_SHAPE;
+    const long long rhs_bytes = (long long)k * n * rhs_bits / 8;
+    if (rhs_bytes >= 0x7fffffffLL) return MC_EINVAL_SHAPE;
+    if ((long long)m_vec * vec_length * (long long)n >= (1LL << 40)) return MC_EINVAL_SHAPE;
+    if (m_vec == 0) return MC_OK;
+    if (!row_offsets || !column_indices || !values || !rhs_matrix || !output_matrix) return MC_EINVAL_ARG;
+    if (config < -1 || config >= kNumSpmmConfigs) return MC_EINVAL_ARG;
+    int cfg = config >= 0 ? config : default_config(lhs_bits, rhs_bits, vec_length, n);
+    if ((n / cpd) % kSpmmConfigs[cfg].lane_dwords != 0) {
+        if (config >= 0) return MC_EINVAL_SHAPE;
+        cfg = 0;
+    }
+
+    SpmmArgs a{};
+    a.m_vec = m_vec;
+    a.n = n;
+    a.k = k;
+    a.row_indices = row_indices;
+    a.row_offsets = row_offsets;
+    a.col_indices = column_indices;
+    a.values = reinterpret_cast<const unsigned char *>(values);
+    a.rhs = reinterpret_cast<const unsigned char *>(rhs_matrix);
+    a.out = output_matrix;
+    a.rhs_bytes = (unsigned)rhs_bytes;
+    const bool sgn = sign == MC_SIGNED;
+    if (lhs_bits == 8 && rhs_bits == 8) return spmm_launch_l8r8(sgn, vec_length, cfg, a, stream);
+    if (lhs_bits == 16 && rhs_bits == 8) return spmm_launch_l16r8(sgn, vec_length, cfg, a, stream);
+    if (lhs_bits == 8 && rhs_bits == 4) return spmm_launch_l8r4(sgn, vec_length, cfg, a, stream);
+    return spmm_launch_l4r4(sgn, vec_length, cfg, a, stream);
+}
+
+}  // namespace
+}  // namespace mc
+
+extern "C" {
+
+mc_status mc_spmm_l8r8(int m_vec, int vec_length, int n, int k, const int *row_indices, const int *row_offsets,
+                       const int *column_indices, const int *values, const int *rhs_matrix, int *output_matrix,
+                       mc_signedness sign, hipStream_t stream) {
+    return mc::spmm_entry(8, 8, sign, -1, m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values,
+                          rhs_matrix, output_matrix, stream);
+}
+
+mc_status mc_spmm_l16r8(int m_vec, int vec_length, int n, int k, const int *row_indices, const int *row_offsets,
+                        const int *column_indices, const int *values, const int *rhs_matrix, int *output_matrix,
+                        mc_signedness sign, hipStream_t stream) {
+    return mc::spmm_entry(16, 8, sign, -1, m_vec, vec_length, n, k, row_indices, row_offsets, column_indices,
+                          values, rhs_matrix, output_matrix, stream);
+}
+
+mc_status mc_spmm_l8r4(int m_vec, int vec_length, int n, int k, const int *row_indices, const int *row_offsets,
+                       const int *column_indices, const int *values, const int *rhs_matrix, int *output_matrix,
+                       mc_signedness sign, hipStream_t stream) {
+    return mc::spmm_entry(8, 4, sign, -1, m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values,
+                          rhs_matrix, output_matrix, stream);
+}
+
+mc_status mc_spmm_l4r4(int m_vec, int vec_length, int n, int k, const int *row_indices, const int *row_offsets,
+                       const int *column_indices, const int *values, const int *rhs_matrix, int *output_matrix,
+                       mc_signedness sign, hipStream_t stream) {
+    return mc::spmm_entry(4, 4, sign, -1, m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values,
+                          rhs_matrix, output_matrix, stream);
+}
+
+int mc_spmm_num_tile_configs(void) { return mc::kNumSpmmConfigs; }
+
+mc_status mc_spmm_get_tile_config(int index, mc_tile_config *out) {
+    if (index < 0 || index >= mc::kNumSpmmConfigs || !out) return MC_EINVAL_ARG;
+    *out = mc::kSpmmConfigs[index];
+    return MC_OK;
+}
+
+mc_status mc_spmm_run(int lhs_bits, int rhs_bits, mc_signedness sign, int config, int m_vec, int vec_length, int n,
+                      int k, const int *row_indices, const int *row_offsets, const int *column_indices,
+                      const int *values, const int *rhs_matrix, int *output_matrix, hipStream_t stream) {
+    return mc::spmm_entry(lhs_bits, rhs_bits, sign, config, m_vec, vec_length, n, k, row_indices, row_offsets,
+                          column_indices, values, rhs_matrix, output_matrix, stream);
+}
+
+mc_status mc_spmm_set_tile_config(int lhs_bits, int rhs_bits, int vec_length, int n, int config) {
+    if (!mc::pair_supported(lhs_bits, rhs_bits)) return MC_EUNSUPPORTED;
+    if (!mc::valid_v(vec_length)) return MC_EINVAL_V;
+    if (config < -1 || config >= mc::kNumSpmmConfigs) return MC_EINVAL_ARG;
+    if (config == 4 && rhs_bits != 8) return MC_EUNSUPPORTED;
+    std::lock_guard<std::mutex> lk(mc::g_cfg_mutex);
+    auto key = std::make_tuple(lhs_bits, rhs_bits, vec_length, n);
+    if (config < 0) mc::g_cfg_table.erase(key);
+    else mc::g_cfg_table[key] = config;
+    return MC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+// Template dispatch for the SpMM kernel family: (V, bits, signedness, tile
+// config) -> one instantiation.  Each pair is instantiated in its own
+// translation unit (spmm_l*.hip) so the build parallelizes.
+#pragma once
+
+#include "mc_internal.hpp"
+#include "spmm_kernel.hpp"
+
+namespace mc {
+
+// Tile configurations: {lane_dwords, tiles_per_block, split_k}.
+constexpr int kNumSpmmConfigs = 6;
+constexpr mc_tile_config kSpmmConfigs[kNumSpmmConfigs] = {
+    {1, 4, 1}, {2, 2, 1}, {1, 2, 2}, {1, 1, 4}, {4, 1, 1}, {2, 1, 2},
+};
+
+template <int V, int AB, int BB, bool SGN, int LW, int TPB, int SK>
+mc_status spmm_launch_one(SpmmArgs a, hipStream_t stream) {
+    constexpr int TILE_N = 16 * LW * (32 / BB);
+    if ((a.n / (32 / BB)) % LW != 0) return MC_EUNSUPPORTED;
+    a.tiles_n = (a.n + TILE_N - 1) / TILE_N;
+    const int groups = (a.tiles_n + TPB - 1) / TPB;
+    const long long blocks = (long long)a.m_vec * groups;
+    if (blocks <= 0) return MC_OK;
+    if (blocks > 0x7fffffffLL) return MC_EINVAL_SHAPE;
+    hipLaunchKernelGGL((spmm_kernel<V, AB, BB, SGN, LW, TPB, SK>), dim3((unsigned)blocks), dim3(64 * TPB * SK), 0,
+                       stream, a);
+    return check_launch();
+}
+
+template <int V, int AB, int BB, bool SGN>
+mc_status spmm_launch_cfg(int cfg, const SpmmArgs &a, hipStream_t s) {
+    switch (cfg) {
+    case 0: return spmm_launch_one<V, AB, BB, SGN, 1, 4, 1>(a, s);
+    case 1: return spmm_launch_one<V, AB, BB, SGN, 2, 2, 1>(a, s);
+    case 2: return spmm_launch_one<V, AB, BB, SGN, 1, 2, 2>(a, s);
+    case 3: return spmm_launch_one<V, AB, BB, SGN, 1, 1, 4>(a, s);
+    case 4:
+        if constexpr (BB == 8) return spmm_launch_one<V, AB, BB, SGN, 4, 1, 1>(a, s);
+        else return MC_EUNSUPPORTED;
+    case 5: return spmm_launch_one<V, AB, BB, SGN, 2, 1, 2>(a, s);
+    default: return MC_EINVAL_ARG;
+    }
+}
+
+template <int AB, int BB>
+mc_status spmm_launch_pair(bool sgn, int V, int cfg, const SpmmArgs &a, hipStream_t s) {
+    if (sgn) {
+        switch (V) {
+        case 2: return spmm_launch_cfg<2, AB, BB, true>(cfg, a, s);
+        case 4: return spmm_launch_cfg<4, AB, BB, true>(cfg, a, s);
+        case 8: return spmm_launch_cfg<8, AB, BB, true>(cfg, a, s);
+        }
+    } else {
+        switch (V) {
+        case 2: return spmm_launch_cfg<2, AB, BB, false>(cfg, a, s);
+        case 4: return spmm_launch_cfg<4, AB, BB, false>(cfg, a, s);
+        case 8: return spmm_launch_cfg<8, AB, BB, false>(cfg, a, s);
+        }
+    }
+    return MC_EINVAL_V;
+}
+
+mc_status spmm_launch_l8r8(bool sgn, int V, int cfg, const SpmmArgs &a, hipStream_t s);
+mc_status spmm_launch_l16r8(bool sgn, int V, int cfg, const SpmmArgs &a, hipStream_t s);
+mc_status spmm_launch_l8r4(bool sgn, int V, int cfg, const SpmmArgs &a, hipStream_t s);
+mc_status spmm_launch_l4r4(bool sgn, int V, int cfg, const SpmmArgs &a, hipStream_t s);
+
+}  // namespace mc

@@ -1,0 +1,382 @@
+// Quantized vector-sparse SpMM for gfx950: C[M x N] (int32) = A * B.
+//
+// A is Magicube's vector-CSR (V x 1 column vectors, rows padded to MMA_K
+// slots, (start, end) pairs), with values in the reference's prepared layout
+// (SpMM/SpMM/spmm_benchmark.cpp:238-377).  B is K x N row-major, packed
+// 4 x int8 or 8 x int4 per int32 with lane 0 in the low bits.
+//
+// Work unit: one wave = one vector row x TILE_N columns x (1/SPLITK) of the
+// row's nonzero vectors.  Per 64-slot k-block every lane
+//   * loads the 16 column indices of its lane group (slots 16g..16g+15),
+//   * gathers LW dwords of each of those 16 B rows with raw buffer loads
+//     (padding slots, index -1, and bad indices fall outside the buffer and
+//     read as zero without touching memory),
+//   * byte-transposes 4x4 blocks with v_perm so every lane holds 16
+//     reduction slots of one column, and issues v_mfma_i32_16x16x64_i8.
+//
+// Operand digits.  The MFMA is signed int8 x int8, so every operand is a sum
+// of int8 digit planes with power-of-two weights; A planes stack as MFMA rows
+// (row = plane*V + v, padded to 16-row tiles), and the planes are recombined
+// by shifted int32 adds in the epilogue.  B digits may carry a constant offset
+// OB (u8 -> b^0x80 = b-128, signed i4 -> nibble^8 = b+8), corrected with
+// OB * (row sum of A), computed on the A operand with v_dot4.
+//
+//   pair    signed                          unsigned
+//   L8-R8   A s8                             A = 16*hi4 + lo4, B = d + 128
+//   L16-R8  A = 256*s8 + 16*u4 + u4          A = 4 nibbles,  B = d + 128
+//   L8-R4   A s8 (from the two nibble        A = 16*hi4 + lo4
+//           planes), B = d - 8
+//   L4-R4   A s4->s8, B = d - 8              A u4, B u4
+//
+// The output is the exact sum modulo 2^32.
+#pragma once
+
+#include "mc_device.hpp"
+
+namespace mc {
+
+struct SpmmArgs {
+    int m_vec;
+    int n;
+    int k;
+    const int *row_indices;     // row swizzle (nullptr: identity)
+    const int *row_offsets;     // 2*m_vec (start, end) pairs
+    const int *col_indices;     // padded with -1; shuffled in 8-groups for 4-bit B
+    const unsigned char *values;
+    const unsigned char *rhs;
+    int *out;
+    int tiles_n;                // ceil(n / TILE_N)
+    unsigned rhs_bytes;         // k * n * BB / 8  (< 2^31)
+};
+
+template <int AB, int BB, bool SGN> struct Pair;
+
+template <bool SGN> struct Pair<8, 8, SGN> {
+    static constexpr int MMA_K = 16, P = SGN ? 1 : 2, OB = SGN ? 0 : 128;
+    __device__ static constexpr int weight(int p) { return SGN ? 1 : (p == 0 ? 16 : 1); }
+};
+template <bool SGN> struct Pair<16, 8, SGN> {
+    static constexpr int MMA_K = 16, P = SGN ? 3 : 4, OB = SGN ? 0 : 128;
+    __device__ static constexpr int weight(int p) {
+        return SGN ? (p == 0 ? 256 : p == 1 ? 16 : 1) : (p == 0 ? 4096 : p == 1 ? 256 : p == 2 ? 16 : 1);
+    }
+};
+template <bool SGN> struct Pair<8, 4, SGN> {
+    static constexpr int MMA_K = 32, P = SGN ? 1 : 2, OB = SGN ? -8 : 0;
+    __device__ static constexpr int weight(int p) { return SGN ? 1 : (p == 0 ? 16 : 1); }
+};
+template <bool SGN> struct Pair<4, 4, SGN> {
+    static constexpr int MMA_K = 32, P = 1, OB = SGN ? -8 : 0;
+    __device__ static constexpr int weight(int) { return 1; }
+};
+
+// A operand of one lane: plane p, vector lane v, reduction slots
+// [slot0, slot0 + 16).  Returns zeros when !ok (buffer offset out of range).
+template <int V, int AB, int BB, bool SGN>
+__device__ __forceinline__ v4i load_a_operand(__amdgpu_buffer_rsrc_t va, int slot0, int p, int v, bool ok) {
+    const u32 OOB = 0x80000000u;
+    if constexpr (BB == 8) {
+        // groups of 16 slots; [V][16] bytes per byte plane
+        const int grp = slot0 >> 4;
+        if constexpr (AB == 8) {
+            const u32 off = ok ? (u32)(grp * 16 * V + 16 * v) : OOB;
+            v4i x = buf_load_b128(va, off);
+            if constexpr (SGN) {
+                return x;
+            } else {
+                const int sh = (p == 0) ? 4 : 0;
+                v4i y;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) y[q] = (int)(((u32)x[q] >> sh) & 0x0F0F0F0Fu);
+                return y;
+            }
+        } else {  // AB == 16: low-byte plane then high-byte plane per group
+            const bool hi = SGN ? (p == 0) : (p < 2);
+            const u32 off = ok ? (u32)(grp * 32 * V + (hi ? 16 * V : 0) + 16 * v) : OOB;
+            v4i x = buf_load_b128(va, off);
+            if (SGN && p == 0) return x;
+            const int sh = SGN ? (p == 1 ? 4 : 0) : ((p & 1) == 0 ? 4 : 0);
+            v4i y;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) y[q] = (int)(((u32)x[q] >> sh) & 0x0F0F0F0Fu);
+            return y;
+        }
+    } else {
+        // groups of 32 slots; the lane's 16 slots are half h of the group.
+        // Slot order follows the 8-group column shuffle (evens then odds),
+        // which is exactly [even nibbles(x0), odd(x0), even(x1), odd(x1)].
+        const int grp = slot0 >> 5, h = (slot0 >> 4) & 1;
+        if constexpr (AB == 4) {
+            const u32 off = ok ? (u32)(grp * 16 * V + 16 * v + 8 * h) : OOB;
+            v2i x = buf_load_b64(va, off);
+            u32 e0 = nib_even((u32)x[0]), o0 = nib_odd((u32)x[0]);
+            u32 e1 = nib_even((u32)x[1]), o1 = nib_odd((u32)x[1]);
+            if constexpr (SGN) { e0 = sext4_bytes(e0); o0 = sext4_bytes(o0); e1 = sext4_bytes(e1); o1 = sext4_bytes(o1); }
+            return v4i{(int)e0, (int)o0, (int)e1, (int)o1};
+        } else {  // AB == 8 with 4-bit B: low-nibble plane then high-nibble plane
+            const u32 base = (u32)(grp * 32 * V + 16 * v + 8 * h);
+            if constexpr (SGN) {
+                v2i lo = buf_load_b64(va, ok ? base : OOB);
+                v2i hi = buf_load_b64(va, ok ? base + 16 * V : OOB);
+                u32 e0 = nib_even((u32)lo[0]) | (nib_even((u32)hi[0]) << 4);
+                u32 o0 = nib_odd((u32)lo[0]) | (nib_odd((u32)hi[0]) << 4);
+                u32 e1 = nib_even((u32)lo[1]) | (nib_even((u32)hi[1]) << 4);
+                u32 o1 = nib_odd((u32)lo[1]) | (nib_odd((u32)hi[1]) << 4);
+                return v4i{(int)e0, (int)o0, (int)e1, (int)o1};
+            } else {
+                v2i x = buf_load_b64(va, ok ? base + (p == 0 ? 16 * V : 0) : OOB);
+                return v4i{(int)nib_even((u32)x[0]), (int)nib_odd((u32)x[0]),
+                           (int)nib_even((u32)x[1]), (int)nib_odd((u32)x[1])};
+            }
+        }
+    }
+}
+
+template <int LW>
+__device__ __forceinline__ void load_b_row(__amdgpu_buffer_rsrc_t rb, u32 off, u32 (&dst)[LW]) {
+    if constexpr (LW == 1) {
+        dst[0] = buf_load_b32(rb, off);
+    } else if constexpr (LW == 2) {
+        v2i x = buf_load_b64(rb, off);
+        dst[0] = (u32)x[0]; dst[1] = (u32)x[1];
+    } else {
+        v4i x = buf_load_b128(rb, off);
+        dst[0] = (u32)x[0]; dst[1] = (u32)x[1]; dst[2] = (u32)x[2]; dst[3] = (u32)x[3];
+    }
+}
+
+template <int V, int AB, int BB, bool SGN, int LW, int TPB, int SPLITK>
+__global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) {
+    using T = Pair<AB, BB, SGN>;
+    constexpr int P = T::P;
+    constexpr int ROWS = P * V;
+    constexpr int RT = (ROWS + 15) / 16;
+    constexpr int CPD = 32 / BB;            // columns per B dword
+    constexpr int NMMA = LW * CPD;          // 16-column MFMA groups per wave
+    constexpr int TILE_N = 16 * NMMA;
+    constexpr int MMA_K = T::MMA_K;
+    constexpr int OB = T::OB;
+    constexpr u32 OOB = 0x80000000u;
+    constexpr int HOME = V < 4 ? V : 4;      // output rows held per home lane
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int tile_local = wave % TPB;
+    const int split = wave / TPB;
+    const int tile_groups = (args.tiles_n + TPB - 1) / TPB;
+    const int row_slot = blockIdx.x / tile_groups;
+    const int tile = (blockIdx.x % tile_groups) * TPB + tile_local;
+
+    const int m = args.row_indices ? args.row_indices[row_slot] : row_slot;
+    const int start = args.row_offsets[2 * m];
+    const int end = args.row_offsets[2 * m + 1];
+    const int aend = start + ((end - start + MMA_K - 1) / MMA_K) * MMA_K;
+    const int nblk = (aend - start + 63) >> 6;
+
+    const int n_dw = args.n / CPD;                        // dwords per B row
+    const int col_dw = tile * (TILE_N / CPD) + r * LW;    // first dword of this lane
+    const bool col_ok = (tile < args.tiles_n) && (col_dw < n_dw);
+    const u32 row_stride = (u32)n_dw * 4u;
+
+    const __amdgpu_buffer_rsrc_t rb = make_rsrc(args.rhs, args.rhs_bytes);
+    const __amdgpu_buffer_rsrc_t ri = make_rsrc(args.col_indices, 0x7FFFFFFFu);
+    const __amdgpu_buffer_rsrc_t ra = make_rsrc(args.values, 0x7FFFFFFFu);
+
+    // plane / vector lane of this lane's A row in each row tile
+    int a_p[RT], a_v[RT];
+    bool a_live[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+        const int R = 16 * t + r;
+        a_live[t] = R < ROWS;
+        a_p[t] = R / V;
+        a_v[t] = R % V;
+    }
+
+    v4i acc[RT][NMMA];
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int j = 0; j < NMMA; ++j) acc[t][j] = v4i{0, 0, 0, 0};
+    int rsum[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) rsum[t] = 0;
+
+    auto load_idx = [&](int kb, v4i (&ix)[4], bool &ok) {
+        const int slot0 = start + kb * 64 + 16 * g;
+        ok = (kb < nblk) && (slot0 < aend);
+        const u32 off = ok ? (u32)slot0 * 4u : OOB;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ix[q] = buf_load_b128(ri, off + 16u * q);
+    };
+    auto load_ab = [&](int kb, const v4i (&ix)[4], bool ok, u32 (&braw)[16][LW], v4i (&aop)[RT]) {
+        const int slot0 = start + kb * 64 + 16 * g;
+        const bool bok = ok && col_ok;
+        const u32 coff = (u32)col_dw * 4u;
+#pragma unroll
+        for (int p = 0; p < 16; ++p) {
+            const int idx = ix[p >> 2][p & 3];
+            const u32 off = (bok && idx >= 0) ? (u32)idx * row_stride + coff : OOB;
+            load_b_row<LW>(rb, off, braw[p]);
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+            aop[t] = load_a_operand<V, AB, BB, SGN>(ra, slot0, a_p[t], a_v[t], ok && a_live[t]);
+    };
+    auto compute = [&](u32 (&braw)[16][LW], const v4i (&aop)[RT]) {
+#pragma unroll
+        for (int d = 0; d < LW; ++d) {
+            v4i bop[CPD];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                u32 o[4];
+                if constexpr (BB == 8) {
+                    // unsigned bytes become signed digits b - 128 (OB = +128)
+                    const u32 x = SGN ? 0u : 0x80808080u;
+                    transpose4x4(braw[4 * q][d] ^ x, braw[4 * q + 1][d] ^ x, braw[4 * q + 2][d] ^ x,
+                                 braw[4 * q + 3][d] ^ x, o);
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) bop[b][q] = (int)o[b];
+                } else {
+                    const u32 x = SGN ? 0x88888888u : 0u;
+                    transpose4x4(braw[4 * q][d] ^ x, braw[4 * q + 1][d] ^ x, braw[4 * q + 2][d] ^ x,
+                                 braw[4 * q + 3][d] ^ x, o);
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        bop[2 * b][q] = (int)nib_even(o[b]);
+                        bop[2 * b + 1][q] = (int)nib_odd(o[b]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int c = 0; c < CPD; ++c) acc[t][d * CPD + c] = mfma_i8(aop[t], bop[c], acc[t][d * CPD + c]);
+        }
+        if constexpr (OB != 0) {
+#pragma unroll
+            for (int t = 0; t < RT; ++t) rsum[t] += sum_bytes(aop[t]);
+        }
+    };
+
+    // Two-stage pipeline: indices two blocks ahead, A/B one block ahead.
+    {
+        v4i ix0[4], ix1[4];
+        bool ok0, ok1;
+        u32 b0[16][LW], b1[16][LW];
+        v4i a0[RT], a1[RT];
+        int kb = split;
+        load_idx(kb, ix0, ok0);
+        load_ab(kb, ix0, ok0, b0, a0);
+        load_idx(kb + SPLITK, ix1, ok1);
+        for (; kb < nblk; kb += 2 * SPLITK) {
+            load_ab(kb + SPLITK, ix1, ok1, b1, a1);
+            load_idx(kb + 2 * SPLITK, ix0, ok0);
+            compute(b0, a0);
+            if (kb + SPLITK >= nblk) break;
+            load_ab(kb + 2 * SPLITK, ix0, ok0, b0, a0);
+            load_idx(kb + 3 * SPLITK, ix1, ok1);
+            compute(b1, a1);
+        }
+    }
+
+    // ---- epilogue: recombine planes (shifted adds), offset correction ----
+    // Output row v lives in home lane group v/4 (element v%4).  Plane p of
+    // row v sits in MFMA row R = p*V + v: tile R/16, lane group (R%16)/4,
+    // element R%4.
+    int rs_home[HOME];
+#pragma unroll
+    for (int i = 0; i < HOME; ++i) rs_home[i] = 0;
+    if constexpr (OB != 0) {
+        int s_full[RT];
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            int s = rsum[t];
+            s += lane_read(s, lane ^ 16);
+            s += lane_read(s, lane ^ 32);
+            s_full[t] = s;  // row 16t + r
+        }
+#pragma unroll
+        for (int i = 0; i < HOME; ++i) {
+            const int v = 4 * g + i;
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const int R = p * V + v;
+                int sv = 0;
+#pragma unroll
+                for (int t = 0; t < RT; ++t) {
+                    const int got = lane_read(s_full[t], R & 15);
+                    if ((R >> 4) == t) sv = got;
+                }
+                rs_home[i] += T::weight(p) * sv;
+            }
+        }
+    }
+
+    int res[NMMA][HOME];
+#pragma unroll
+    for (int j = 0; j < NMMA; ++j)
+#pragma unroll
+        for (int i = 0; i < HOME; ++i) {
+            int vsum = OB * rs_home[i];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                constexpr int dummy = 0;
+                (void)dummy;
+                int t, dg, di;
+                if constexpr (V == 8) { t = p / 2; dg = 2 * (p % 2); di = 0; }
+                else if constexpr (V == 4) { t = p / 4; dg = p % 4; di = 0; }
+                else { t = p / 8; dg = (p % 8) / 2; di = 2 * (p % 2); }
+                int x = acc[t][j][i + di];
+                if (dg != 0) x = lane_down(x, 16 * dg);
+                vsum += T::weight(p) * x;
+            }
+            res[j][i] = vsum;
+        }
+
+    const bool home = g < (V + 3) / 4;
+    if constexpr (SPLITK > 1) {
+        // waves with split > 0 hand their partial tile to split 0 via LDS
+        __shared__ int part[(SPLITK > 1 ? SPLITK - 1 : 1) * TPB][V][TILE_N];
+        if (split > 0 && home) {
+#pragma unroll
+            for (int j = 0; j < NMMA; ++j) {
+                const int col = (r * LW + j / CPD) * CPD + j % CPD;
+#pragma unroll
+                for (int i = 0; i < HOME; ++i) part[(split - 1) * TPB + tile_local][4 * g + i][col] = res[j][i];
+            }
+        }
+        __syncthreads();
+        if (split > 0) return;
+        if (home) {
+#pragma unroll
+            for (int s = 1; s < SPLITK; ++s)
+#pragma unroll
+                for (int j = 0; j < NMMA; ++j) {
+                    const int col = (r * LW + j / CPD) * CPD + j % CPD;
+#pragma unroll
+                    for (int i = 0; i < HOME; ++i) res[j][i] += part[(s - 1) * TPB + tile_local][4 * g + i][col];
+                }
+        }
+    }
+
+    if (!home || !col_ok) return;
+    // lane r owns dwords [col_dw, col_dw + LW): CPD consecutive columns each
+#pragma unroll
+    for (int i = 0; i < HOME; ++i) {
+        const int v = 4 * g + i;
+        int *orow = args.out + (size_t)(m * V + v) * (size_t)args.n + (size_t)col_dw * CPD;
+#pragma unroll
+        for (int d = 0; d < LW; ++d) {
+#pragma unroll
+            for (int c4 = 0; c4 < CPD; c4 += 4) {
+                v4i o = v4i{res[d * CPD + c4][i], res[d * CPD + c4 + 1][i], res[d * CPD + c4 + 2][i],
+                            res[d * CPD + c4 + 3][i]};
+                __builtin_nontemporal_store(o, reinterpret_cast<v4i *>(orow + d * CPD + c4));
+            }
+        }
+    }
+}
+
+}  // namespace mc

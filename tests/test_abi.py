@@ -1,0 +1,99 @@
+"""The C-ABI library loads, exports every symbol the headers declare, and
+rejects bad arguments before any device work (CPU-only checks)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from magicube_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(mc_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_loads(mclib):
+    assert mclib.mc_version() >= 1
+
+
+def test_every_declared_symbol_is_exported(mclib):
+    names = declared_functions()
+    assert len(names) >= 10
+    missing = [n for n in names if not hasattr(mclib, n)]
+    assert not missing, missing
+
+
+def test_every_declared_symbol_has_a_binding():
+    names = declared_functions()
+    unbound = sorted(n for n in names if n not in _lib.exported_symbols())
+    assert not unbound, unbound
+
+
+def test_status_strings(mclib):
+    for s in range(7):
+        assert _lib.status_string(s)
+
+
+NULL = None
+
+
+def _spmm(mclib, name, **kw):
+    args = dict(m_vec=4, V=8, n=64, k=64, ri=NULL, ro=ctypes.c_void_p(16), ci=ctypes.c_void_p(16),
+                vals=ctypes.c_void_p(16), rhs=ctypes.c_void_p(16), out=ctypes.c_void_p(16), sign=0, stream=NULL)
+    args.update(kw)
+    fn = getattr(mclib, name)
+    return fn(args["m_vec"], args["V"], args["n"], args["k"], args["ri"], args["ro"], args["ci"], args["vals"],
+              args["rhs"], args["out"], args["sign"], args["stream"])
+
+
+@pytest.mark.parametrize("name", ["mc_spmm_l8r8", "mc_spmm_l16r8", "mc_spmm_l8r4", "mc_spmm_l4r4"])
+def test_spmm_validation(mclib, name):
+    assert _spmm(mclib, name, V=3) == _lib.MC_EINVAL_V
+    assert _spmm(mclib, name, V=16) == _lib.MC_EINVAL_V
+    assert _spmm(mclib, name, n=0) == _lib.MC_EINVAL_SHAPE
+    assert _spmm(mclib, name, n=6) == _lib.MC_EINVAL_SHAPE          # not a multiple of 4 / 8
+    assert _spmm(mclib, name, k=-1) == _lib.MC_EINVAL_SHAPE
+    assert _spmm(mclib, name, sign=7) == _lib.MC_EINVAL_ARG
+    assert _spmm(mclib, name, ro=NULL) == _lib.MC_EINVAL_ARG
+    assert _spmm(mclib, name, out=NULL) == _lib.MC_EINVAL_ARG
+    assert _spmm(mclib, name, k=1 << 20, n=1 << 16) == _lib.MC_EINVAL_SHAPE  # B over 2 GiB
+    # an empty matrix is a no-op that succeeds without launching
+    assert _spmm(mclib, name, m_vec=0) == _lib.MC_OK
+
+
+def test_spmm_run_validation(mclib):
+    P = ctypes.c_void_p(16)
+    assert mclib.mc_spmm_run(12, 8, 0, -1, 4, 8, 64, 64, None, P, P, P, P, P, None) == _lib.MC_EUNSUPPORTED
+    assert mclib.mc_spmm_run(8, 8, 0, 99, 4, 8, 64, 64, None, P, P, P, P, P, None) == _lib.MC_EINVAL_ARG
+    # config 1 gathers two dwords per lane: n/4 must be even
+    assert mclib.mc_spmm_run(8, 8, 0, 1, 4, 8, 68, 64, None, P, P, P, P, P, None) == _lib.MC_EINVAL_SHAPE
+
+
+def test_tile_config_table(mclib):
+    from magicube_amd import spmm
+    cfgs = spmm.tile_configs()
+    assert len(cfgs) == mclib.mc_spmm_num_tile_configs() >= 4
+    assert all(c[0] in (1, 2, 4) and c[1] >= 1 and c[2] >= 1 for c in cfgs)
+    assert mclib.mc_spmm_set_tile_config(8, 8, 8, 512, 2) == _lib.MC_OK
+    assert mclib.mc_spmm_set_tile_config(8, 8, 8, 512, -1) == _lib.MC_OK
+    assert mclib.mc_spmm_set_tile_config(8, 8, 5, 512, 0) == _lib.MC_EINVAL_V
+    assert mclib.mc_spmm_set_tile_config(8, 4, 8, 512, 4) == _lib.MC_EUNSUPPORTED
+
+
+def test_format_validation(mclib):
+    assert mclib.mc_format_row_alignment(8, 8) == 16
+    assert mclib.mc_format_row_alignment(8, 4) == 32
+    assert mclib.mc_format_row_alignment(4, 4) == 32
+    assert mclib.mc_format_align_rows(-1, None, 16, None) == -1

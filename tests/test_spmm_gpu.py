@@ -1,0 +1,145 @@
+"""SpMM parity on the GPU through the C ABI: bit-exact against the oracle.
+
+Exact comparisons at sizes the oracle finishes in about a second (including
+the 512x512 CPU-path workload and the M=K=4096, N=512 headline), plus
+size-independent checksums C w == A (B w) (mod 2^32) over the whole
+precision sweep (pairs x V x sparsity x N at M=K=4096).
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from magicube_amd import formats, spmm, synthetic
+from tests.spmm_cases import checksum_expected, checksum_got, make_case
+
+pytestmark = pytest.mark.gpu
+
+PAIRS = [(8, 8), (16, 8), (8, 4), (4, 4)]
+
+
+def run_case(case, config=None, row_order="sorted", prefill=0x5A5A5A5A, stream=None):
+    dev = spmm.DeviceSpmm.upload(case.prep)
+    if row_order == "identity":
+        dev.row_indices = torch.arange(case.m_vec, dtype=torch.int32, device="cuda")
+    elif row_order == "null":
+        dev.row_indices = None
+    rhs = torch.from_numpy(case.rhs_packed).cuda()
+    out = torch.full((case.m_vec * case.V, case.N), prefill, dtype=torch.int32, device="cuda")
+    sign = spmm.SIGNED if case.signed else spmm.UNSIGNED
+    print(f"case L{case.lhs_bits}-R{case.rhs_bits} V={case.V} signed={case.signed} m_vec={case.m_vec} "
+          f"K={case.K} N={case.N} cfg={config}", flush=True)
+    dev.matmul(rhs, case.N, out=out, sign=sign, config=config, stream=stream)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def oracle_result(orc, case):
+    return orc.spmm_plain(case.m_vec, case.V, case.N, case.csr.row_offsets, case.csr.col_indices, case.a,
+                          case.b)
+
+
+@pytest.mark.parametrize("signed", [True, False])
+@pytest.mark.parametrize("V", [2, 4, 8])
+@pytest.mark.parametrize("pair", PAIRS)
+def test_spmm_small_exact(orc, pair, V, signed):
+    case = make_case(*pair, V, signed, M=96 * V // 2, K=384, N=256, sparsity=0.8, seed=11 * V + pair[0])
+    got = run_case(case)
+    assert np.array_equal(got, oracle_result(orc, case))
+
+
+@pytest.mark.parametrize("config", range(6))
+@pytest.mark.parametrize("pair", PAIRS)
+def test_spmm_every_tile_config(orc, pair, config):
+    if config == 4 and pair[1] == 4:
+        pytest.skip("config 4 (4 dwords per lane) is built for 8-bit B only")
+    for V in (2, 8):
+        case = make_case(*pair, V, True, M=64 * V, K=512, N=512, sparsity=0.85, seed=config * 7 + V)
+        got = run_case(case, config=config)
+        assert np.array_equal(got, oracle_result(orc, case)), f"V={V}"
+
+
+@pytest.mark.parametrize("pair", PAIRS)
+def test_spmm_ragged_n_and_empty_rows(orc, pair):
+    # N not a multiple of the tile; rows with 0, 1 and many nonzeros
+    cpd = 32 // pair[1]
+    N = 13 * cpd * 3
+    case = make_case(*pair, 4, True, M=40 * 4, K=300, N=N, sparsity=0.9, seed=5)
+    ro = case.csr.row_offsets
+    assert (np.diff(ro) >= 0).all()
+    got = run_case(case)
+    assert np.array_equal(got, oracle_result(orc, case))
+
+
+def test_spmm_fully_overwrites_output_and_handles_empty_matrix(orc):
+    csr = formats.VectorCSR(8, 64, 8, np.zeros(9, np.int32), np.zeros(0, np.int32))
+    prep = formats.prepare_spmm(csr, np.zeros(0, np.uint8), 8, 8)
+    dev = spmm.DeviceSpmm.upload(prep)
+    rhs = torch.ones(64 * 64 // 4, dtype=torch.int32, device="cuda")
+    out = torch.full((64, 64), 12345, dtype=torch.int32, device="cuda")
+    dev.matmul(rhs, 64, out=out)
+    torch.cuda.synchronize()
+    assert (out == 0).all()
+
+
+@pytest.mark.parametrize("order", ["sorted", "identity", "null"])
+def test_spmm_row_order_does_not_change_results(orc, order):
+    case = make_case(8, 8, 8, True, M=512, K=512, N=128, sparsity=0.9, seed=3)
+    assert np.array_equal(run_case(case, row_order=order), oracle_result(orc, case))
+
+
+def test_spmm_side_stream(orc):
+    case = make_case(8, 4, 8, True, M=256, K=512, N=256, sparsity=0.9, seed=4)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        got = run_case(case, stream=s)
+    assert np.array_equal(got, oracle_result(orc, case))
+
+
+@pytest.mark.parametrize("signed", [True, False])
+def test_cpu_path_workload_512(orc, signed):
+    # 512x512 A, V=8, 90% sparse with 51 vectors per block, B 512x128 int8
+    case = make_case(8, 8, 8, signed, M=512, K=512, N=128, sparsity=0.9, seed=2024, mask="uniform")
+    assert np.all(np.diff(case.csr.row_offsets) == 51)
+    assert np.array_equal(run_case(case), oracle_result(orc, case))
+
+
+def test_reference_generator_data_unsigned(orc):
+    # Values from the reference's own generator (MakeDenseMatrix<long long> /
+    # <int>) checked against the literal compute_ref_integers restatement.
+    M, K, N, V = 512, 512, 128, 8
+    csr = synthetic.uniform_block_mask(M, K, V, 0.9, seed=1)
+    nnz = csr.nnz_vec
+    vals = orc.make_dense_matrix(8, nnz)            # one long long per V=8 vector
+    B = orc.make_dense_matrix(4, K * N // 4)
+    C_ref, _ = orc.spmm_ref_check(vals, 8, B, M, K, N, 8, 8, 8, V, csr.row_offsets, csr.col_indices,
+                                  csr.m_vec, 1)
+    prep = formats.prepare_spmm(csr, vals.view(np.uint8), 8, 8)
+    dev = spmm.DeviceSpmm.upload(prep)
+    out = torch.empty((M, N), dtype=torch.int32, device="cuda")
+    dev.matmul(torch.from_numpy(B).cuda(), N, out=out, sign=spmm.UNSIGNED)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert np.array_equal(got, C_ref)
+    assert (got > 0).sum() == M * N   # the published counter = M*N invariant
+
+
+@pytest.mark.parametrize("signed", [True, False])
+def test_headline_exact(orc, signed):
+    # M=K=4096, N=512, V=8, 90% sparse (Poisson per block), L8-R8
+    case = make_case(8, 8, 8, signed, M=4096, K=4096, N=512, sparsity=0.9, seed=7)
+    assert np.array_equal(run_case(case), oracle_result(orc, case))
+
+
+SWEEP = [(pair, V, s, N) for pair in PAIRS for V in (2, 4, 8) for s in (0.7, 0.9, 0.98) for N in (128, 256, 512)]
+
+
+@pytest.mark.parametrize("pair,V,s,N", SWEEP)
+def test_sweep_checksums(pair, V, s, N):
+    case = make_case(*pair, V, True, M=4096, K=4096, N=N, sparsity=s, seed=int(s * 100) + V + N)
+    got = run_case(case)
+    rng = np.random.default_rng(0)
+    for _ in range(2):
+        w = rng.integers(0, 1 << 10, size=N)
+        assert np.array_equal(checksum_got(got, w), checksum_expected(case, w))
