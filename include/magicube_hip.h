@@ -80,13 +80,15 @@ mc_status mc_spmm_l4r4(int m_vec, int vec_length, int n, int k, const int *row_i
 /* Tile configurations of the SpMM kernel (an index into a fixed table):
  * lane_dwords B dwords gathered per lane per row, tiles_per_block column
  * tiles per workgroup, split_k waves sharing one (row, tile) over the
- * nonzero vectors.  No reference counterpart (the reference fixes its tile
+ * nonzero vectors, depth k-blocks of gathers in flight per wave.  No
+ * reference counterpart (the reference fixes its tile
  * per template, SpMM/SpMM/src/wmma_spmm.cu:2137-2153); used by the
  * autotuner. */
 typedef struct mc_tile_config {
     int lane_dwords;
     int tiles_per_block;
     int split_k;
+    int depth;
 } mc_tile_config;
 int mc_spmm_num_tile_configs(void);
 mc_status mc_spmm_get_tile_config(int index, mc_tile_config *out);

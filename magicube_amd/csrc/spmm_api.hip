@@ -21,10 +21,14 @@ int default_config(int lhs_bits, int rhs_bits, int V, int n) {
     (void)lhs_bits; (void)V;
     const int cpd = 32 / rhs_bits;
     const int n_dw = n / cpd;
-    // Wider lanes when the row allows it; a single config otherwise.
-    if (rhs_bits == 8 && n_dw % 2 == 0 && n >= 256) return 1;
-    return 0;
+    if (rhs_bits == 8 && n_dw % 4 == 0 && n >= 512) return 2;
+    if (n_dw % 2 == 0) return 1;
+    return 4;
 }
+
+// Four dwords per lane are built for 8-bit B only (4-bit B would hold 32
+// accumulator tiles per wave).
+bool config_supported(int cfg, int rhs_bits) { return rhs_bits == 8 || kSpmmConfigs[cfg].lane_dwords < 4; }
 
 bool pair_supported(int lhs_bits, int rhs_bits) {
     return (lhs_bits == 8 && rhs_bits == 8) || (lhs_bits == 16 && rhs_bits == 8) ||
@@ -47,9 +51,10 @@ mc_status spmm_entry(int lhs_bits, int rhs_bits, mc_signedness sign, int config,
     if (!row_offsets || !column_indices || !values || !rhs_matrix || !output_matrix) return MC_EINVAL_ARG;
     if (config < -1 || config >= kNumSpmmConfigs) return MC_EINVAL_ARG;
     int cfg = config >= 0 ? config : default_config(lhs_bits, rhs_bits, vec_length, n);
+    if (!config_supported(cfg, rhs_bits)) return MC_EUNSUPPORTED;
     if ((n / cpd) % kSpmmConfigs[cfg].lane_dwords != 0) {
         if (config >= 0) return MC_EINVAL_SHAPE;
-        cfg = 0;
+        cfg = 4;   // one dword per lane fits every n
     }
 
     SpmmArgs a{};
@@ -122,7 +127,7 @@ mc_status mc_spmm_set_tile_config(int lhs_bits, int rhs_bits, int vec_length, in
     if (!mc::pair_supported(lhs_bits, rhs_bits)) return MC_EUNSUPPORTED;
     if (!mc::valid_v(vec_length)) return MC_EINVAL_V;
     if (config < -1 || config >= mc::kNumSpmmConfigs) return MC_EINVAL_ARG;
-    if (config == 4 && rhs_bits != 8) return MC_EUNSUPPORTED;
+    if (config >= 0 && !mc::config_supported(config, rhs_bits)) return MC_EUNSUPPORTED;
     std::lock_guard<std::mutex> lk(mc::g_cfg_mutex);
     auto key = std::make_tuple(lhs_bits, rhs_bits, vec_length, n);
     if (config < 0) mc::g_cfg_table.erase(key);

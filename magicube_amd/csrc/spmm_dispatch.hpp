@@ -8,13 +8,14 @@
 
 namespace mc {
 
-// Tile configurations: {lane_dwords, tiles_per_block, split_k}.
-constexpr int kNumSpmmConfigs = 6;
+// Tile configurations: {lane_dwords, tiles_per_block, split_k, depth}.
+constexpr int kNumSpmmConfigs = 8;
 constexpr mc_tile_config kSpmmConfigs[kNumSpmmConfigs] = {
-    {1, 4, 1}, {2, 2, 1}, {1, 2, 2}, {1, 1, 4}, {4, 1, 1}, {2, 1, 2},
+    {2, 4, 1, 2}, {2, 4, 2, 2}, {4, 2, 2, 2}, {4, 2, 4, 2},
+    {1, 4, 2, 3}, {2, 2, 2, 2}, {4, 1, 4, 2}, {1, 8, 1, 3},
 };
 
-template <int V, int AB, int BB, bool SGN, int LW, int TPB, int SK>
+template <int V, int AB, int BB, bool SGN, int LW, int TPB, int SK, int D>
 mc_status spmm_launch_one(SpmmArgs a, hipStream_t stream) {
     constexpr int TILE_N = 16 * LW * (32 / BB);
     if ((a.n / (32 / BB)) % LW != 0) return MC_EUNSUPPORTED;
@@ -23,7 +24,7 @@ mc_status spmm_launch_one(SpmmArgs a, hipStream_t stream) {
     const long long blocks = (long long)a.m_vec * groups;
     if (blocks <= 0) return MC_OK;
     if (blocks > 0x7fffffffLL) return MC_EINVAL_SHAPE;
-    hipLaunchKernelGGL((spmm_kernel<V, AB, BB, SGN, LW, TPB, SK>), dim3((unsigned)blocks), dim3(64 * TPB * SK), 0,
+    hipLaunchKernelGGL((spmm_kernel<V, AB, BB, SGN, LW, TPB, SK, D>), dim3((unsigned)blocks), dim3(64 * TPB * SK), 0,
                        stream, a);
     return check_launch();
 }
@@ -31,14 +32,20 @@ mc_status spmm_launch_one(SpmmArgs a, hipStream_t stream) {
 template <int V, int AB, int BB, bool SGN>
 mc_status spmm_launch_cfg(int cfg, const SpmmArgs &a, hipStream_t s) {
     switch (cfg) {
-    case 0: return spmm_launch_one<V, AB, BB, SGN, 1, 4, 1>(a, s);
-    case 1: return spmm_launch_one<V, AB, BB, SGN, 2, 2, 1>(a, s);
-    case 2: return spmm_launch_one<V, AB, BB, SGN, 1, 2, 2>(a, s);
-    case 3: return spmm_launch_one<V, AB, BB, SGN, 1, 1, 4>(a, s);
-    case 4:
-        if constexpr (BB == 8) return spmm_launch_one<V, AB, BB, SGN, 4, 1, 1>(a, s);
+    case 0: return spmm_launch_one<V, AB, BB, SGN, 2, 4, 1, 2>(a, s);
+    case 1: return spmm_launch_one<V, AB, BB, SGN, 2, 4, 2, 2>(a, s);
+    case 2:
+        if constexpr (BB == 8) return spmm_launch_one<V, AB, BB, SGN, 4, 2, 2, 2>(a, s);
         else return MC_EUNSUPPORTED;
-    case 5: return spmm_launch_one<V, AB, BB, SGN, 2, 1, 2>(a, s);
+    case 3:
+        if constexpr (BB == 8) return spmm_launch_one<V, AB, BB, SGN, 4, 2, 4, 2>(a, s);
+        else return MC_EUNSUPPORTED;
+    case 4: return spmm_launch_one<V, AB, BB, SGN, 1, 4, 2, 3>(a, s);
+    case 5: return spmm_launch_one<V, AB, BB, SGN, 2, 2, 2, 2>(a, s);
+    case 6:
+        if constexpr (BB == 8) return spmm_launch_one<V, AB, BB, SGN, 4, 1, 4, 2>(a, s);
+        else return MC_EUNSUPPORTED;
+    case 7: return spmm_launch_one<V, AB, BB, SGN, 1, 8, 1, 3>(a, s);
     default: return MC_EINVAL_ARG;
     }
 }

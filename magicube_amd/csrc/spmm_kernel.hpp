@@ -145,7 +145,10 @@ __device__ __forceinline__ void load_b_row(__amdgpu_buffer_rsrc_t rb, u32 off, u
     }
 }
 
-template <int V, int AB, int BB, bool SGN, int LW, int TPB, int SPLITK>
+// Gather offsets of one row chunk, staged in LDS per workgroup.
+constexpr int kStageSlots = 1024;
+
+template <int V, int AB, int BB, bool SGN, int LW, int TPB, int SPLITK, int DEPTH>
 __global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) {
     using T = Pair<AB, BB, SGN>;
     constexpr int P = T::P;
@@ -156,8 +159,10 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) 
     constexpr int TILE_N = 16 * NMMA;
     constexpr int MMA_K = T::MMA_K;
     constexpr int OB = T::OB;
+    constexpr int NT = 64 * TPB * SPLITK;
     constexpr u32 OOB = 0x80000000u;
     constexpr int HOME = V < 4 ? V : 4;      // output rows held per home lane
+    __shared__ __attribute__((aligned(16))) u32 s_off[kStageSlots];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -172,15 +177,15 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) 
     const int start = args.row_offsets[2 * m];
     const int end = args.row_offsets[2 * m + 1];
     const int aend = start + ((end - start + MMA_K - 1) / MMA_K) * MMA_K;
-    const int nblk = (aend - start + 63) >> 6;
 
     const int n_dw = args.n / CPD;                        // dwords per B row
     const int col_dw = tile * (TILE_N / CPD) + r * LW;    // first dword of this lane
     const bool col_ok = (tile < args.tiles_n) && (col_dw < n_dw);
     const u32 row_stride = (u32)n_dw * 4u;
+    // lanes past the last column add OOB: real rows then read out of range
+    const u32 coff = col_ok ? (u32)col_dw * 4u : OOB;
 
     const __amdgpu_buffer_rsrc_t rb = make_rsrc(args.rhs, args.rhs_bytes);
-    const __amdgpu_buffer_rsrc_t ri = make_rsrc(args.col_indices, 0x7FFFFFFFu);
     const __amdgpu_buffer_rsrc_t ra = make_rsrc(args.values, 0x7FFFFFFFu);
 
     // plane / vector lane of this lane's A row in each row tile
@@ -203,27 +208,6 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) 
 #pragma unroll
     for (int t = 0; t < RT; ++t) rsum[t] = 0;
 
-    auto load_idx = [&](int kb, v4i (&ix)[4], bool &ok) {
-        const int slot0 = start + kb * 64 + 16 * g;
-        ok = (kb < nblk) && (slot0 < aend);
-        const u32 off = ok ? (u32)slot0 * 4u : OOB;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) ix[q] = buf_load_b128(ri, off + 16u * q);
-    };
-    auto load_ab = [&](int kb, const v4i (&ix)[4], bool ok, u32 (&braw)[16][LW], v4i (&aop)[RT]) {
-        const int slot0 = start + kb * 64 + 16 * g;
-        const bool bok = ok && col_ok;
-        const u32 coff = (u32)col_dw * 4u;
-#pragma unroll
-        for (int p = 0; p < 16; ++p) {
-            const int idx = ix[p >> 2][p & 3];
-            const u32 off = (bok && idx >= 0) ? (u32)idx * row_stride + coff : OOB;
-            load_b_row<LW>(rb, off, braw[p]);
-        }
-#pragma unroll
-        for (int t = 0; t < RT; ++t)
-            aop[t] = load_a_operand<V, AB, BB, SGN>(ra, slot0, a_p[t], a_v[t], ok && a_live[t]);
-    };
     auto compute = [&](u32 (&braw)[16][LW], const v4i (&aop)[RT]) {
 #pragma unroll
         for (int d = 0; d < LW; ++d) {
@@ -239,6 +223,7 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) 
 #pragma unroll
                     for (int b = 0; b < 4; ++b) bop[b][q] = (int)o[b];
                 } else {
+                    // signed nibbles become digits b + 8 (OB = -8)
                     const u32 x = SGN ? 0x88888888u : 0u;
                     transpose4x4(braw[4 * q][d] ^ x, braw[4 * q + 1][d] ^ x, braw[4 * q + 2][d] ^ x,
                                  braw[4 * q + 3][d] ^ x, o);
@@ -260,24 +245,57 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) 
         }
     };
 
-    // Two-stage pipeline: indices two blocks ahead, A/B one block ahead.
-    {
-        v4i ix0[4], ix1[4];
-        bool ok0, ok1;
-        u32 b0[16][LW], b1[16][LW];
-        v4i a0[RT], a1[RT];
-        int kb = split;
-        load_idx(kb, ix0, ok0);
-        load_ab(kb, ix0, ok0, b0, a0);
-        load_idx(kb + SPLITK, ix1, ok1);
-        for (; kb < nblk; kb += 2 * SPLITK) {
-            load_ab(kb + SPLITK, ix1, ok1, b1, a1);
-            load_idx(kb + 2 * SPLITK, ix0, ok0);
-            compute(b0, a0);
-            if (kb + SPLITK >= nblk) break;
-            load_ab(kb + 2 * SPLITK, ix0, ok0, b0, a0);
-            load_idx(kb + 3 * SPLITK, ix1, ok1);
-            compute(b1, a1);
+    // The row's slots are processed in chunks of kStageSlots.  Per chunk the
+    // workgroup stages each slot's B byte offset (index * row stride, OOB for
+    // padding) in LDS once; each wave then keeps DEPTH k-blocks of gathers in
+    // flight, reading its lane group's 16 offsets with LDS broadcasts.
+    for (int c0 = start; c0 < aend; c0 += kStageSlots) {
+        const int cn = min(aend - c0, kStageSlots);
+        const int cblk = (cn + 63) >> 6;
+        __syncthreads();
+        for (int i = threadIdx.x; i < cblk * 64; i += NT) {
+            u32 o = OOB;
+            if (i < cn) {
+                const int idx = args.col_indices[c0 + i];
+                if (idx >= 0) o = (u32)idx * row_stride;
+            }
+            s_off[i] = o;
+        }
+        __syncthreads();
+
+        auto issue = [&](int kb, u32 (&br)[16][LW], v4i (&ar)[RT]) {
+            const bool ok = kb < cblk;
+            const int lb = ok ? kb : 0;
+            const int slot0 = c0 + lb * 64 + 16 * g;
+            const bool aok = ok && (slot0 < aend);
+            const v4i *so = reinterpret_cast<const v4i *>(s_off + lb * 64 + 16 * g);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const v4i o4 = so[q];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const u32 off = ok ? (u32)o4[e] + coff : OOB;
+                    load_b_row<LW>(rb, off, br[4 * q + e]);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+                ar[t] = load_a_operand<V, AB, BB, SGN>(ra, slot0, a_p[t], a_v[t], aok && a_live[t]);
+        };
+
+        u32 ring[DEPTH][16][LW];
+        v4i aring[DEPTH][RT];
+#pragma unroll
+        for (int d = 0; d < DEPTH; ++d) issue(split + d * SPLITK, ring[d], aring[d]);
+        for (int base = split; base < cblk; base += DEPTH * SPLITK) {
+#pragma unroll
+            for (int d = 0; d < DEPTH; ++d) {
+                const int kb = base + d * SPLITK;
+                if (kb < cblk) {
+                    compute(ring[d], aring[d]);
+                    issue(kb + DEPTH * SPLITK, ring[d], aring[d]);
+                }
+            }
         }
     }
 

@@ -79,17 +79,19 @@ def test_spmm_run_validation(mclib):
     assert mclib.mc_spmm_run(8, 8, 0, 99, 4, 8, 64, 64, None, P, P, P, P, P, None) == _lib.MC_EINVAL_ARG
     # config 1 gathers two dwords per lane: n/4 must be even
     assert mclib.mc_spmm_run(8, 8, 0, 1, 4, 8, 68, 64, None, P, P, P, P, P, None) == _lib.MC_EINVAL_SHAPE
+    # four dwords per lane are built for 8-bit B only
+    assert mclib.mc_spmm_run(8, 4, 0, 2, 4, 8, 512, 64, None, P, P, P, P, P, None) == _lib.MC_EUNSUPPORTED
 
 
 def test_tile_config_table(mclib):
     from magicube_amd import spmm
     cfgs = spmm.tile_configs()
     assert len(cfgs) == mclib.mc_spmm_num_tile_configs() >= 4
-    assert all(c[0] in (1, 2, 4) and c[1] >= 1 and c[2] >= 1 for c in cfgs)
+    assert all(c[0] in (1, 2, 4) and c[1] >= 1 and c[2] >= 1 and c[3] >= 1 for c in cfgs)
     assert mclib.mc_spmm_set_tile_config(8, 8, 8, 512, 2) == _lib.MC_OK
     assert mclib.mc_spmm_set_tile_config(8, 8, 8, 512, -1) == _lib.MC_OK
     assert mclib.mc_spmm_set_tile_config(8, 8, 5, 512, 0) == _lib.MC_EINVAL_V
-    assert mclib.mc_spmm_set_tile_config(8, 4, 8, 512, 4) == _lib.MC_EUNSUPPORTED
+    assert mclib.mc_spmm_set_tile_config(8, 4, 8, 512, 2) == _lib.MC_EUNSUPPORTED
 
 
 def test_format_validation(mclib):
