@@ -58,7 +58,7 @@ int mc_last_hip_error(void);  /* hipError_t of the last MC_EHIP in this thread *
  * (SpMM/SpMM/include/wmma_spmm.cuh:15-70, launchers
  * SpMM/SpMM/src/wmma_spmm.cu:2091-2383): same arguments in the same order,
  * plus the lane interpretation and a stream.  n must be a multiple of 4
- * (8-bit B) or 8 (4-bit B); k*n*bits(B)/8 < 2^31.  row_indices may be NULL
+ * (8-bit B) or 8 (4-bit B); k*n*bits(B)/8 < 2^30.  row_indices may be NULL
  * (identity order). */
 mc_status mc_spmm_l8r8(int m_vec, int vec_length, int n, int k, const int *row_indices,
                        const int *row_offsets, const int *column_indices, const int *values,

@@ -147,6 +147,10 @@ __device__ __forceinline__ void load_b_row(__amdgpu_buffer_rsrc_t rb, u32 off, u
 
 // Gather offsets of one row chunk, staged in LDS per workgroup.
 constexpr int kStageSlots = 1024;
+// B is limited to 1 GiB: offsets of real rows are < 2^30, and adding this
+// constant (to a staged offset or to a padding marker of the same value)
+// always lands in [2^30, 2^31), outside the buffer.
+constexpr u32 kOobAdd = 0x40000000u;
 
 template <int V, int AB, int BB, bool SGN, int LW, int TPB, int SPLITK, int DEPTH>
 __global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) {
@@ -182,8 +186,8 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) 
     const int col_dw = tile * (TILE_N / CPD) + r * LW;    // first dword of this lane
     const bool col_ok = (tile < args.tiles_n) && (col_dw < n_dw);
     const u32 row_stride = (u32)n_dw * 4u;
-    // lanes past the last column add OOB: real rows then read out of range
-    const u32 coff = col_ok ? (u32)col_dw * 4u : OOB;
+    // lanes past the last column add kOobAdd: real rows then read out of range
+    const u32 coff = col_ok ? (u32)col_dw * 4u : kOobAdd;
 
     const __amdgpu_buffer_rsrc_t rb = make_rsrc(args.rhs, args.rhs_bytes);
     const __amdgpu_buffer_rsrc_t ra = make_rsrc(args.values, 0x7FFFFFFFu);
@@ -254,7 +258,7 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) 
         const int cblk = (cn + 63) >> 6;
         __syncthreads();
         for (int i = threadIdx.x; i < cblk * 64; i += NT) {
-            u32 o = OOB;
+            u32 o = kOobAdd;
             if (i < cn) {
                 const int idx = args.col_indices[c0 + i];
                 if (idx >= 0) o = (u32)idx * row_stride;
@@ -263,9 +267,13 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) 
         }
         __syncthreads();
 
+        // Branch-free issue: the LDS block index is clamped into the chunk and
+        // validity rides on one additive offset (B is < 1 GiB, so adding
+        // kOobAdd to any staged offset lands outside the buffer, no wrap).
         auto issue = [&](int kb, u32 (&br)[16][LW], v4i (&ar)[RT]) {
             const bool ok = kb < cblk;
-            const int lb = ok ? kb : 0;
+            const int lb = ok ? kb : cblk - 1;
+            const u32 kadd = ok ? coff : kOobAdd;
             const int slot0 = c0 + lb * 64 + 16 * g;
             const bool aok = ok && (slot0 < aend);
             const v4i *so = reinterpret_cast<const v4i *>(s_off + lb * 64 + 16 * g);
@@ -273,10 +281,7 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) 
             for (int q = 0; q < 4; ++q) {
                 const v4i o4 = so[q];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const u32 off = ok ? (u32)o4[e] + coff : OOB;
-                    load_b_row<LW>(rb, off, br[4 * q + e]);
-                }
+                for (int e = 0; e < 4; ++e) load_b_row<LW>(rb, (u32)o4[e] + kadd, br[4 * q + e]);
             }
 #pragma unroll
             for (int t = 0; t < RT; ++t)

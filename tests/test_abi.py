@@ -68,7 +68,8 @@ def test_spmm_validation(mclib, name):
     assert _spmm(mclib, name, sign=7) == _lib.MC_EINVAL_ARG
     assert _spmm(mclib, name, ro=NULL) == _lib.MC_EINVAL_ARG
     assert _spmm(mclib, name, out=NULL) == _lib.MC_EINVAL_ARG
-    assert _spmm(mclib, name, k=1 << 20, n=1 << 16) == _lib.MC_EINVAL_SHAPE  # B over 2 GiB
+    assert _spmm(mclib, name, k=1 << 20, n=1 << 16) == _lib.MC_EINVAL_SHAPE  # B over 1 GiB
+    assert _spmm(mclib, name, k=1 << 18, n=1 << 14) == _lib.MC_EINVAL_SHAPE
     # an empty matrix is a no-op that succeeds without launching
     assert _spmm(mclib, name, m_vec=0) == _lib.MC_OK
 
