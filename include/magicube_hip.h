@@ -102,6 +102,30 @@ mc_status mc_spmm_run(int lhs_bits, int rhs_bits, mc_signedness sign, int config
  * vec_length, n) — the autotuner's result; -1 restores the default. */
 mc_status mc_spmm_set_tile_config(int lhs_bits, int rhs_bits, int vec_length, int n, int config);
 
+/* ---- SDDMM ----------------------------------------------------------------
+ * Replace sddmm::wmmaSddmm_8b / _4b (SDDMM/SDDMM/include/wmma_sddmm.cuh:7-32,
+ * launchers SDDMM/SDDMM/src/wmma_sddmm.cu:376-465): same arguments in the
+ * same order (note k before n), plus the lane interpretation and a stream.
+ *   out[j][v] = sum_k lhs[(i*V+v)][k] * rhs[col_j][k]
+ * lhs is (m_vec*vec_length) x k, rhs n x k, both row-major packed; k is 64,
+ * 128 or 256; row starts are multiples of 8.  Every slot of
+ * [start, start + roundup8(end - start)) is written, padding slots with 0, in
+ * the reference layout [(j/8)][v][j%8] (SDDMM/SDDMM/sddmm_benchmark.cpp:55). */
+mc_status mc_sddmm_l8r8(int m_vec, int k, int n, const int *row_indices, const int *row_offsets,
+                        const int *col_indices, const int *lhs_matrix, const int *rhs_matrix,
+                        int *output_values, int vec_length, mc_signedness sign, hipStream_t stream);
+mc_status mc_sddmm_l4r4(int m_vec, int k, int n, const int *row_indices, const int *row_offsets,
+                        const int *col_indices, const int *lhs_matrix, const int *rhs_matrix,
+                        int *output_values, int vec_length, mc_signedness sign, hipStream_t stream);
+/* Output layouts of mc_sddmm_run. */
+#define MC_SDDMM_OUT_REFERENCE 0   /* [(j/8)][v][j%8] */
+#define MC_SDDMM_OUT_VECTOR 1      /* [j][v] (vector-major, the attention layout) */
+/* Generic SDDMM: bits in {4, 8} for both operands, explicit output layout. */
+mc_status mc_sddmm_run(int bits, mc_signedness sign, int layout, int m_vec, int k, int n,
+                       const int *row_indices, const int *row_offsets, const int *col_indices,
+                       const int *lhs_matrix, const int *rhs_matrix, int *output_values,
+                       int vec_length, hipStream_t stream);
+
 /* ---- Host-side format preparation (no device work) ----------------------
  * Builds the reference layouts from a plain vector-CSR (row_offsets has
  * m_vec+1 entries, col_indices nnz_vec entries, values nnz_vec packed

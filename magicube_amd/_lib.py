@@ -42,6 +42,9 @@ _SIGS = {
     "mc_format_prepare_values": (_I, [_I, _I, _I, _I, _P, _P, _I, _P, _P]),
     "mc_format_sorted_rows": (_I, [_I, _P, _P]),
 }
+_SIGS["mc_sddmm_l8r8"] = (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _I, _P])
+_SIGS["mc_sddmm_l4r4"] = (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _I, _P])
+_SIGS["mc_sddmm_run"] = (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P])
 for _name in ("mc_spmm_l8r8", "mc_spmm_l16r8", "mc_spmm_l8r4", "mc_spmm_l4r4"):
     _SIGS[_name] = (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P])
 

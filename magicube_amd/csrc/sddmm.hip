@@ -1,0 +1,102 @@
+// C ABI of the SDDMM family: validation and dispatch.
+#include "mc_internal.hpp"
+#include "sddmm_kernel.hpp"
+
+namespace mc {
+namespace {
+
+constexpr int kSddmmWaves = 8;
+
+template <int V, int BITS, bool SGN, int NKC, bool REF>
+mc_status sddmm_launch(const SddmmArgs &a, hipStream_t s) {
+    constexpr int TPW = NKC >= 4 ? 1 : 4 / NKC;
+    hipLaunchKernelGGL((sddmm_kernel<V, BITS, SGN, NKC, kSddmmWaves, TPW, REF>), dim3((unsigned)a.m_vec),
+                       dim3(64 * kSddmmWaves), 0, s, a);
+    return check_launch();
+}
+
+template <int V, int BITS, bool SGN, bool REF>
+mc_status sddmm_launch_k(int nkc, const SddmmArgs &a, hipStream_t s) {
+    switch (nkc) {
+    case 1: return sddmm_launch<V, BITS, SGN, 1, REF>(a, s);
+    case 2: return sddmm_launch<V, BITS, SGN, 2, REF>(a, s);
+    case 4: return sddmm_launch<V, BITS, SGN, 4, REF>(a, s);
+    default: return MC_EUNSUPPORTED;
+    }
+}
+
+template <int BITS, bool SGN, bool REF>
+mc_status sddmm_launch_v(int V, int nkc, const SddmmArgs &a, hipStream_t s) {
+    switch (V) {
+    case 2: return sddmm_launch_k<2, BITS, SGN, REF>(nkc, a, s);
+    case 4: return sddmm_launch_k<4, BITS, SGN, REF>(nkc, a, s);
+    case 8: return sddmm_launch_k<8, BITS, SGN, REF>(nkc, a, s);
+    default: return MC_EINVAL_V;
+    }
+}
+
+template <int BITS>
+mc_status sddmm_launch_all(bool sgn, bool ref, int V, int nkc, const SddmmArgs &a, hipStream_t s) {
+    if (sgn) return ref ? sddmm_launch_v<BITS, true, true>(V, nkc, a, s) : sddmm_launch_v<BITS, true, false>(V, nkc, a, s);
+    return ref ? sddmm_launch_v<BITS, false, true>(V, nkc, a, s) : sddmm_launch_v<BITS, false, false>(V, nkc, a, s);
+}
+
+mc_status sddmm_entry(int bits, mc_signedness sign, int layout, int m_vec, int k, int n, const int *row_indices,
+                      const int *row_offsets, const int *col_indices, const int *lhs_matrix, const int *rhs_matrix,
+                      int *output_values, int vec_length, hipStream_t stream) {
+    if (bits != 8 && bits != 4) return MC_EUNSUPPORTED;
+    if (sign != MC_SIGNED && sign != MC_UNSIGNED) return MC_EINVAL_ARG;
+    if (layout != MC_SDDMM_OUT_REFERENCE && layout != MC_SDDMM_OUT_VECTOR) return MC_EINVAL_ARG;
+    if (!valid_v(vec_length)) return MC_EINVAL_V;
+    if (m_vec < 0 || n <= 0 || k <= 0 || k % 64 != 0) return MC_EINVAL_SHAPE;
+    const int nkc = k / 64;
+    if (nkc != 1 && nkc != 2 && nkc != 4) return MC_EUNSUPPORTED;
+    const long long lhs_bytes = (long long)m_vec * vec_length * k * bits / 8;
+    const long long rhs_bytes = (long long)n * k * bits / 8;
+    if (lhs_bytes >= (1LL << 30) || rhs_bytes >= (1LL << 30)) return MC_EINVAL_SHAPE;
+    if (m_vec == 0) return MC_OK;
+    if (!row_offsets || !col_indices || !lhs_matrix || !rhs_matrix || !output_values) return MC_EINVAL_ARG;
+    SddmmArgs a{};
+    a.m_vec = m_vec;
+    a.k = k;
+    a.n = n;
+    a.row_indices = row_indices;
+    a.row_offsets = row_offsets;
+    a.col_indices = col_indices;
+    a.lhs = reinterpret_cast<const unsigned char *>(lhs_matrix);
+    a.rhs = reinterpret_cast<const unsigned char *>(rhs_matrix);
+    a.out = output_values;
+    a.lhs_bytes = (unsigned)lhs_bytes;
+    a.rhs_bytes = (unsigned)rhs_bytes;
+    const bool sgn = sign == MC_SIGNED, ref = layout == MC_SDDMM_OUT_REFERENCE;
+    return bits == 8 ? sddmm_launch_all<8>(sgn, ref, vec_length, nkc, a, stream)
+                     : sddmm_launch_all<4>(sgn, ref, vec_length, nkc, a, stream);
+}
+
+}  // namespace
+}  // namespace mc
+
+extern "C" {
+
+mc_status mc_sddmm_l8r8(int m_vec, int k, int n, const int *row_indices, const int *row_offsets,
+                        const int *col_indices, const int *lhs_matrix, const int *rhs_matrix, int *output_values,
+                        int vec_length, mc_signedness sign, hipStream_t stream) {
+    return mc::sddmm_entry(8, sign, MC_SDDMM_OUT_REFERENCE, m_vec, k, n, row_indices, row_offsets, col_indices,
+                           lhs_matrix, rhs_matrix, output_values, vec_length, stream);
+}
+
+mc_status mc_sddmm_l4r4(int m_vec, int k, int n, const int *row_indices, const int *row_offsets,
+                        const int *col_indices, const int *lhs_matrix, const int *rhs_matrix, int *output_values,
+                        int vec_length, mc_signedness sign, hipStream_t stream) {
+    return mc::sddmm_entry(4, sign, MC_SDDMM_OUT_REFERENCE, m_vec, k, n, row_indices, row_offsets, col_indices,
+                           lhs_matrix, rhs_matrix, output_values, vec_length, stream);
+}
+
+mc_status mc_sddmm_run(int bits, mc_signedness sign, int layout, int m_vec, int k, int n, const int *row_indices,
+                       const int *row_offsets, const int *col_indices, const int *lhs_matrix,
+                       const int *rhs_matrix, int *output_values, int vec_length, hipStream_t stream) {
+    return mc::sddmm_entry(bits, sign, layout, m_vec, k, n, row_indices, row_offsets, col_indices, lhs_matrix,
+                           rhs_matrix, output_values, vec_length, stream);
+}
+
+}  // extern "C"

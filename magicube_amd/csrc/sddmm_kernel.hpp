@@ -1,0 +1,206 @@
+// Quantized vector-sparse SDDMM for gfx950:
+//   out[j][v] = sum_k lhs[i*V + v][k] * rhs[col_j][k]   for every nonzero slot j
+// of vector row i (Magicube's wmmaSddmm, SDDMM/SDDMM/src/wmma_sddmm.cu:15-198).
+//
+// lhs is M x K and rhs is N x K, row-major, packed 4 x int8 or 8 x int4 per
+// int32 (lane 0 low).  Both operands are contiguous along K, so MFMA operands
+// are plain 16-byte (8-bit) or 8-byte (4-bit) loads: lane (r, g) takes the
+// 16 k-values [64*kc + 16g, +16) of lhs row r (A operand) and of the gathered
+// rhs row of nonzero r (B operand) — no transposes.  One MFMA covers 16
+// nonzeros x V rows x 64 k.
+//
+// Digits as in the SpMM kernel: unsigned 8-bit lhs splits into two nibble
+// planes (MFMA rows plane*V + v); unsigned 8-bit rhs uses d = b ^ 0x80 with
+// offset +128, signed 4-bit rhs d = b ^ 8 with offset -8; the offset is
+// corrected with the lhs row sum over K.  Results are exact modulo 2^32.
+//
+// Output slots: every slot of [start, roundup8(end)) is written (padding
+// slots get 0), in the reference layout [(j/8)][v][j%8] (OUT_REF) or
+// vector-major [j][v].
+#pragma once
+
+#include "mc_device.hpp"
+
+namespace mc {
+
+struct SddmmArgs {
+    int m_vec;
+    int k;
+    int n;
+    const int *row_indices;    // may be null (identity)
+    const int *row_offsets;    // 2*m_vec (start, end)
+    const int *col_indices;    // rhs row per slot, -1 in padding
+    const unsigned char *lhs;
+    const unsigned char *rhs;
+    int *out;
+    unsigned lhs_bytes;
+    unsigned rhs_bytes;        // < 2^30
+};
+
+template <int BITS, bool SGN> struct SddmmDigits;
+template <bool SGN> struct SddmmDigits<8, SGN> {
+    static constexpr int P = SGN ? 1 : 2, OB = SGN ? 0 : 128;
+    __device__ static constexpr int weight(int p) { return SGN ? 1 : (p == 0 ? 16 : 1); }
+};
+template <bool SGN> struct SddmmDigits<4, SGN> {
+    static constexpr int P = 1, OB = SGN ? -8 : 0;
+    __device__ static constexpr int weight(int) { return 1; }
+};
+
+// 16 k-values of one packed row at k offset k0 (multiple of 16) as an int8
+// operand.  8-bit: 16 bytes.  4-bit: 8 bytes expanded to [even nibbles(x0),
+// odd(x0), even(x1), odd(x1)] — the same k order for both operands.
+template <int BITS>
+__device__ __forceinline__ v4i load_k16(__amdgpu_buffer_rsrc_t r, u32 off_bytes) {
+    if constexpr (BITS == 8) {
+        return buf_load_b128(r, off_bytes);
+    } else {
+        v2i x = buf_load_b64(r, off_bytes);
+        return v4i{(int)x[0], (int)x[1], 0, 0};   // expanded by the caller
+    }
+}
+
+__device__ __forceinline__ v4i expand_nibbles(v4i x, bool sext, u32 xor_mask) {
+    u32 a = (u32)x[0] ^ xor_mask, b = (u32)x[1] ^ xor_mask;
+    u32 e0 = nib_even(a), o0 = nib_odd(a), e1 = nib_even(b), o1 = nib_odd(b);
+    if (sext) { e0 = sext4_bytes(e0); o0 = sext4_bytes(o0); e1 = sext4_bytes(e1); o1 = sext4_bytes(o1); }
+    return v4i{(int)e0, (int)o0, (int)e1, (int)o1};
+}
+
+constexpr int kSddmmStage = 1024;
+constexpr u32 kSddmmOob = 0x40000000u;
+
+template <int V, int BITS, bool SGN, int NKC, int W, int TPW, bool OUT_REF>
+__global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
+    using D = SddmmDigits<BITS, SGN>;
+    constexpr int P = D::P;
+    constexpr int ROWS = P * V;
+    constexpr int OB = D::OB;
+    constexpr int HOME = V < 4 ? V : 4;
+    constexpr int KB = 64 * BITS / 8;          // bytes of one 64-k chunk
+    __shared__ __attribute__((aligned(16))) u32 s_off[kSddmmStage];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int m = args.row_indices ? args.row_indices[blockIdx.x] : blockIdx.x;
+    const int start = args.row_offsets[2 * m];
+    const int end = args.row_offsets[2 * m + 1];
+    const int wend = start + ((end - start + 7) & ~7);   // written slots
+
+    const u32 row_bytes = (u32)args.k * BITS / 8;
+    const __amdgpu_buffer_rsrc_t rl = make_rsrc(args.lhs, args.lhs_bytes);
+    const __amdgpu_buffer_rsrc_t rr = make_rsrc(args.rhs, args.rhs_bytes);
+
+    // A operands (lhs rows of this vector row) for every k chunk, and their
+    // digit row sums for the offset correction.
+    const bool a_live = r < ROWS;
+    const int a_p = r / V, a_v = r % V;
+    v4i aop[NKC];
+    int rsum = 0;
+#pragma unroll
+    for (int kc = 0; kc < NKC; ++kc) {
+        const u32 off = a_live ? (u32)(m * V + a_v) * row_bytes + (u32)(kc * KB + 16 * g * BITS / 8) : 0x80000000u;
+        v4i x = load_k16<BITS>(rl, off);
+        if constexpr (BITS == 8) {
+            if constexpr (!SGN) {
+                const int sh = a_p == 0 ? 4 : 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) x[q] = (int)(((u32)x[q] >> sh) & 0x0F0F0F0Fu);
+            }
+        } else {
+            x = expand_nibbles(x, SGN, 0u);
+        }
+        aop[kc] = x;
+        if constexpr (OB != 0) rsum += sum_bytes(x);
+    }
+    int rs_home[HOME];
+#pragma unroll
+    for (int i = 0; i < HOME; ++i) rs_home[i] = 0;
+    if constexpr (OB != 0) {
+        int s = rsum;
+        s += lane_read(s, lane ^ 16);
+        s += lane_read(s, lane ^ 32);   // full row sum of MFMA row r
+#pragma unroll
+        for (int i = 0; i < HOME; ++i) {
+            const int v = 4 * g + i;
+#pragma unroll
+            for (int p = 0; p < P; ++p) rs_home[i] += D::weight(p) * lane_read(s, (p * V + v) & 15);
+        }
+    }
+
+    const int ntiles_total = (wend - start + 15) >> 4;
+    for (int c0 = start; c0 < wend; c0 += kSddmmStage) {
+        const int cn = min(wend - c0, kSddmmStage);
+        __syncthreads();
+        for (int i = threadIdx.x; i < ((cn + 15) & ~15); i += 64 * W) {
+            u32 o = kSddmmOob;
+            if (c0 + i < end) {
+                const int idx = args.col_indices[c0 + i];
+                if (idx >= 0) o = (u32)idx * row_bytes;
+            }
+            s_off[i] = o;
+        }
+        __syncthreads();
+        const int ctiles = (cn + 15) >> 4;
+        (void)ntiles_total;
+        // wave w handles tile groups w, w + W, ... of TPW tiles each
+        for (int tg = wave * TPW; tg < ctiles; tg += W * TPW) {
+            v4i bop[TPW][NKC];
+#pragma unroll
+            for (int t = 0; t < TPW; ++t) {
+                const int tile = min(tg + t, ctiles - 1);
+                const u32 add = (tg + t < ctiles) ? 0u : kSddmmOob;
+                const u32 o = s_off[tile * 16 + r] + add;
+#pragma unroll
+                for (int kc = 0; kc < NKC; ++kc) {
+                    v4i x = load_k16<BITS>(rr, o + (u32)(kc * KB + 16 * g * BITS / 8));
+                    if constexpr (BITS == 8) {
+                        if constexpr (!SGN) {
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) x[q] = (int)((u32)x[q] ^ 0x80808080u);
+                        }
+                    } else {
+                        x = expand_nibbles(x, false, SGN ? 0x88888888u : 0u);
+                    }
+                    bop[t][kc] = x;
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < TPW; ++t) {
+                v4i acc = v4i{0, 0, 0, 0};
+#pragma unroll
+                for (int kc = 0; kc < NKC; ++kc) acc = mfma_i8(aop[kc], bop[t][kc], acc);
+                // recombine digit planes into the home lanes (rows v = 4g + i)
+                int res[HOME];
+#pragma unroll
+                for (int i = 0; i < HOME; ++i) {
+                    int vsum = OB * rs_home[i];
+#pragma unroll
+                    for (int p = 0; p < P; ++p) {
+                        int dg, di;
+                        if constexpr (V == 8) { dg = 2 * (p % 2); di = 0; }
+                        else if constexpr (V == 4) { dg = p % 4; di = 0; }
+                        else { dg = (p % 8) / 2; di = 2 * (p % 2); }
+                        int x = acc[i + di];
+                        if (dg != 0) x = lane_down(x, 16 * dg);
+                        vsum += D::weight(p) * x;
+                    }
+                    res[i] = vsum;
+                }
+                const int j = c0 + (tg + t) * 16 + r;
+                if (g < (V + 3) / 4 && tg + t < ctiles && j < wend) {
+#pragma unroll
+                    for (int i = 0; i < HOME; ++i) {
+                        const int v = 4 * g + i;
+                        const int val = j < end ? res[i] : 0;
+                        const size_t o = OUT_REF ? (size_t)(j >> 3) * 8 * V + 8 * v + (j & 7) : (size_t)j * V + v;
+                        args.out[o] = val;
+                    }
+                }
+            }
+        }
+    }
+}
+
+}  // namespace mc

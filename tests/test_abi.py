@@ -100,3 +100,16 @@ def test_format_validation(mclib):
     assert mclib.mc_format_row_alignment(8, 4) == 32
     assert mclib.mc_format_row_alignment(4, 4) == 32
     assert mclib.mc_format_align_rows(-1, None, 16, None) == -1
+
+
+def test_sddmm_validation(mclib):
+    p = ctypes.c_void_p(16)
+    for fn in (mclib.mc_sddmm_l8r8, mclib.mc_sddmm_l4r4):
+        assert fn(4, 100, 64, None, p, p, p, p, p, 8, 0, None) == _lib.MC_EINVAL_SHAPE   # k % 64
+        assert fn(4, 64, 64, None, p, p, p, p, p, 3, 0, None) == _lib.MC_EINVAL_V
+        assert fn(4, 512, 64, None, p, p, p, p, p, 8, 0, None) == _lib.MC_EUNSUPPORTED   # k not built
+        assert fn(4, 64, 64, None, None, p, p, p, p, 8, 0, None) == _lib.MC_EINVAL_ARG
+        assert fn(4, 64, 64, None, p, p, p, p, p, 8, 5, None) == _lib.MC_EINVAL_ARG
+        assert fn(0, 64, 64, None, p, p, p, p, p, 8, 0, None) == _lib.MC_OK
+    assert mclib.mc_sddmm_run(8, 0, 7, 4, 64, 64, None, p, p, p, p, p, 8, None) == _lib.MC_EINVAL_ARG
+    assert mclib.mc_sddmm_run(16, 0, 0, 4, 64, 64, None, p, p, p, p, p, 8, None) == _lib.MC_EUNSUPPORTED
