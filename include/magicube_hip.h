@@ -126,6 +126,85 @@ mc_status mc_sddmm_run(int bits, mc_signedness sign, int layout, int m_vec, int 
                        const int *lhs_matrix, const int *rhs_matrix, int *output_values,
                        int vec_length, hipStream_t stream);
 
+/* ---- Quantized sparse attention ------------------------------------------
+ * Layer semantics of spMultiheadAttention
+ * (end2end_eval/sparse_transformer_magicube/spattention.py:182-216) per
+ * (sample, head) matrix of sequence length L and head dim 64, mask rows
+ * aligned to 16 slots:
+ *   Qq, Kq, Vq = trunc(clamp(x * scale))          (quantization_kernel.cu:46-74)
+ *                or rint(clamp(x * scale)) with MC_ROUND_NEAREST
+ *   S = Qq Kq^T at the mask (int32, exact)
+ *   s = float(S) * alpha     alpha = 1 / (scale_q * scale_k * sqrt(64))
+ *   P, O by the requantization mode:
+ *     MC_P_REFERENCE  P = trunc(min(exp(s - max) / sum * p_scale, 127)),
+ *                     O = float(P Vq) * o_scale           (q_softmax_kernel.cu:132-147)
+ *     MC_P_ROW_SCALED P = rint(min(exp(s - max) * p_scale, 127)),
+ *                     O = (float(P Vq) * o_scale) * (1 / sum)
+ * The softmax statistics are accumulated over 64-slot chunks in a fixed
+ * order, shared by every kernel below, so the one-op-per-launch path and the
+ * fused batched path produce identical int32 S, int8 P and int32 P Vq. */
+#define MC_P_REFERENCE 0
+#define MC_P_ROW_SCALED 1
+#define MC_ROUND_TRUNC 0
+#define MC_ROUND_NEAREST 1
+
+/* One quantization job: count fp16 values -> int8 bytes or packed int4. */
+typedef struct mc_quant_job {
+    const void *x;
+    long long count;
+    float scale;
+    void *out;
+} mc_quant_job;
+/* Replaces sptrans bquantization (src/cuda/quantization.cpp:3-20): jobs is a
+ * device array of njobs jobs; max_count bounds every job's count. */
+mc_status mc_attn_quantize_batched(const mc_quant_job *jobs, int njobs, int bits, int rounding,
+                                   long long max_count, hipStream_t stream);
+/* max |x| of an fp16 tensor, as float bits combined into *out_bits (device,
+ * zero-initialised by the caller) with atomicMax.  No reference counterpart
+ * (the reference uses a fixed scale, spattention.py:51). */
+mc_status mc_attn_absmax(const void *x_fp16, long long count, unsigned *out_bits, hipStream_t stream);
+/* Replaces q_bcsr_softmax (src/cuda/q_softmax.cpp:26-52) for one matrix:
+ * s_vals is the int32 SDDMM output in vector-major layout [slot][V]; P is
+ * written in the SpMM A layout (16-slot groups of [V][16] bytes); row_inv_l
+ * (optional, m_vec*V floats) receives 1/sum per row. */
+mc_status mc_attn_softmax_requant(int m_vec, int vec_length, const int *row_indices,
+                                  const int *row_offsets, const int *s_vals, float alpha,
+                                  float p_scale, int p_mode, void *p_out, float *row_inv_l,
+                                  hipStream_t stream);
+/* out[r][c] = (float(in[r][c]) * scale) * row_scale[r] (row_scale may be NULL):
+ * the dequantizing epilogue of the reference's bspmm (deq_spmm_kernel.cu:645-670). */
+mc_status mc_attn_dequant(const int *in, int rows, int cols, float scale, const float *row_scale,
+                          float *out, hipStream_t stream);
+
+/* One attention matrix of the fused batched launch (device-resident
+ * descriptor).  q, k: [L][64] int8 (or int4 packed [L][32] bytes); v: [L][64]
+ * int8; o: [L][64] float.  o_int (int32 P Vq), p_out (P in the SpMM A layout)
+ * and s_out (int32 S, [slot][V]) are optional debug outputs. */
+typedef struct mc_attn_matrix {
+    int L;
+    int m_vec;
+    const int *row_offsets;
+    const int *col_indices;
+    const void *q;
+    const void *k;
+    const void *v;
+    float alpha;
+    float o_scale;
+    float *o;
+    int *o_int;
+    unsigned char *p_out;
+    int *s_out;
+} mc_attn_matrix;
+/* Fused quantized sparse attention over many matrices in one persistent
+ * launch.  items is a device array of n_items (matrix index, vector row)
+ * int pairs, processed by grid_waves waves (0: one wave per item) in list
+ * order with a stride of grid_waves; any order gives the same results.  No
+ * reference counterpart: the reference launches each op separately with one
+ * mask shared by the batch (spattention.py:189-210). */
+mc_status mc_attn_forward(const mc_attn_matrix *matrices, const int *items, int n_items, int qk_bits,
+                          int vec_length, float p_scale, int p_mode, int grid_waves,
+                          hipStream_t stream);
+
 /* ---- Host-side format preparation (no device work) ----------------------
  * Builds the reference layouts from a plain vector-CSR (row_offsets has
  * m_vec+1 entries, col_indices nnz_vec entries, values nnz_vec packed

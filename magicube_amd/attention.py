@@ -1,0 +1,278 @@
+"""Quantized sparse attention (host mirror of the reference's spMultiheadAttention
+core, end2end_eval/sparse_transformer_magicube/spattention.py:182-216).
+
+An :class:`AttentionBatch` holds any number of (sample, head) matrices of
+uneven sequence length, each with its own vector-sparse mask, resident in
+HBM: fp16 inputs, quantized operands, masks, outputs and the device
+descriptors of the fused launch.  It runs
+
+* :meth:`AttentionBatch.forward` — one batched quantize launch per operand
+  width plus ONE persistent fused launch over a longest-first work list of
+  (matrix, vector row) items (capturable in a hipGraph), or
+* :meth:`AttentionBatch.forward_plain` — the one-op-per-launch pipeline per
+  matrix (quantize x3, SDDMM, softmax + requantize, SpMM, dequantize).
+
+Both give identical int32 logits, int8 probabilities and int32 P V.
+Reference-named op mirrors (``bquantization``, ``q_bcsr_softmax`` ...) are
+at the end of the module.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib, formats
+from .spmm import _dptr, _stream
+
+HEAD_DIM = 64
+P_REFERENCE = 0
+P_ROW_SCALED = 1
+ROUND_TRUNC = 0
+ROUND_NEAREST = 1
+SIGNED = _lib.MC_SIGNED
+
+
+class _AttnMatrixC(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_int), ("m_vec", ctypes.c_int), ("row_offsets", ctypes.c_void_p),
+                ("col_indices", ctypes.c_void_p), ("q", ctypes.c_void_p), ("k", ctypes.c_void_p),
+                ("v", ctypes.c_void_p), ("alpha", ctypes.c_float), ("o_scale", ctypes.c_float),
+                ("o", ctypes.c_void_p), ("o_int", ctypes.c_void_p), ("p_out", ctypes.c_void_p),
+                ("s_out", ctypes.c_void_p)]
+
+
+class _QuantJobC(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_void_p), ("count", ctypes.c_longlong), ("scale", ctypes.c_float),
+                ("out", ctypes.c_void_p)]
+
+
+def _struct_tensor(structs, device) -> torch.Tensor:
+    raw = b"".join(bytes(s) for s in structs)
+    return torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
+
+
+@dataclass
+class AttentionMatrix:
+    """One (sample, head) problem: fp16 q, k, v of shape [L, 64] and a V-row mask."""
+
+    q: torch.Tensor
+    k: torch.Tensor
+    v: torch.Tensor
+    mask: formats.VectorCSR
+
+    @property
+    def L(self) -> int:
+        return int(self.q.shape[0])
+
+
+@dataclass
+class _Dev:
+    L: int
+    m_vec: int
+    nnz: int
+    aligned_num: int
+    pairs: np.ndarray
+    row_offsets: torch.Tensor
+    col_indices: torch.Tensor
+    row_indices: torch.Tensor
+    q16: torch.Tensor
+    k16: torch.Tensor
+    v16: torch.Tensor
+    qq: torch.Tensor
+    kq: torch.Tensor
+    vq: torch.Tensor
+    o: torch.Tensor
+    scales: tuple
+    alpha: float
+    o_scale: float
+    o_int: torch.Tensor | None = None
+    p_out: torch.Tensor | None = None
+    s_out: torch.Tensor | None = None
+    # plain-path work buffers (allocated on first use)
+    work: dict = field(default_factory=dict)
+
+
+def tensor_scales(tensors, stream=None) -> list[float]:
+    """127 / max|x| per fp16 tensor, max computed on the GPU in one pass each."""
+    lib = _lib.lib()
+    dev = tensors[0].device
+    bits = torch.zeros(len(tensors), dtype=torch.int32, device=dev)
+    for i, t in enumerate(tensors):
+        _lib.check(lib.mc_attn_absmax(_dptr(t), t.numel(), ctypes.c_void_p(bits.data_ptr() + 4 * i),
+                                      _stream(stream)), "mc_attn_absmax")
+    amax = bits.cpu().view(torch.float32).numpy()
+    return [127.0 / float(a) if a > 0 else 1.0 for a in amax]
+
+
+class AttentionBatch:
+    """Device-resident quantized sparse attention over uneven matrices."""
+
+    def __init__(self, mats: list[AttentionMatrix], qk_bits: int = 8, V: int = 8, p_mode: int = P_ROW_SCALED,
+                 p_scale: float = 127.0, scales: list[tuple] | None = None, debug: bool = False,
+                 rounding: int | None = None, device="cuda"):
+        if qk_bits not in (8, 4):
+            raise ValueError("qk_bits must be 8 or 4")
+        self.qk_bits, self.V, self.p_mode, self.p_scale = qk_bits, V, p_mode, float(p_scale)
+        # the reference truncates (quantization_kernel.cu:65-69); the row-scaled layer rounds
+        self.rounding = rounding if rounding is not None else (ROUND_TRUNC if p_mode == P_REFERENCE else ROUND_NEAREST)
+        self.device = torch.device(device)
+        if scales is None:
+            flat = tensor_scales([t for m in mats for t in (m.q, m.k, m.v)])
+            scales = [tuple(flat[3 * i:3 * i + 3]) for i in range(len(mats))]
+        self.mats: list[_Dev] = []
+        items = []
+        for e, (m, sc) in enumerate(zip(mats, scales)):
+            if m.q.shape[1] != HEAD_DIM or m.mask.V != V or m.mask.m_vec * V != m.L:
+                raise ValueError("matrix shapes do not match the batch")
+            pairs, an = formats.align_rows(m.mask.row_offsets, 16)
+            cols = formats.align_cols(m.mask.row_offsets, m.mask.col_indices, pairs, an, shuffle=False)
+            order = formats.sorted_rows(m.mask.row_offsets)
+            qk_row = HEAD_DIM * qk_bits // 8
+            sq, sk, sv = (float(x) for x in sc)
+            alpha = float(np.float32(1.0) / (np.float32(sq) * np.float32(sk) * np.float32(math.sqrt(HEAD_DIM))))
+            o_scale = float(np.float32(1.0) / (np.float32(sv) * np.float32(self.p_scale)))
+            d = _Dev(L=m.L, m_vec=m.mask.m_vec, nnz=m.mask.nnz_vec, aligned_num=an, pairs=pairs,
+                     row_offsets=torch.from_numpy(pairs if pairs.size else np.zeros(2, np.int32)).to(self.device),
+                     col_indices=torch.from_numpy(cols if cols.size else np.full(16, -1, np.int32)).to(self.device),
+                     row_indices=torch.from_numpy(order).to(self.device),
+                     q16=m.q.contiguous(), k16=m.k.contiguous(), v16=m.v.contiguous(),
+                     qq=torch.empty(max(m.L * qk_row, 16), dtype=torch.uint8, device=self.device),
+                     kq=torch.empty(max(m.L * qk_row, 16), dtype=torch.uint8, device=self.device),
+                     vq=torch.empty(max(m.L * HEAD_DIM, 16), dtype=torch.uint8, device=self.device),
+                     o=torch.empty((m.L, HEAD_DIM), dtype=torch.float32, device=self.device),
+                     scales=(sq, sk, sv), alpha=alpha, o_scale=o_scale)
+            if debug:
+                d.o_int = torch.empty((m.L, HEAD_DIM), dtype=torch.int32, device=self.device)
+                d.p_out = torch.zeros(max(an * V, 16), dtype=torch.uint8, device=self.device)
+                d.s_out = torch.zeros(max(an * V, 16), dtype=torch.int32, device=self.device)
+            self.mats.append(d)
+            nnz_rows = np.diff(m.mask.row_offsets)
+            for i in range(m.mask.m_vec):
+                items.append((int(nnz_rows[i]), e, i))
+        # longest rows first: the strided persistent walk then balances like LPT
+        items.sort(key=lambda t: -t[0])
+        self.n_items = len(items)
+        it = np.array([(e, i) for _, e, i in items], dtype=np.int32).reshape(-1, 2)
+        self.items = torch.from_numpy(it if it.size else np.zeros((1, 2), np.int32)).to(self.device)
+        self.desc = _struct_tensor([self._desc(d) for d in self.mats], self.device)
+        qk_jobs = [_QuantJobC(x.data_ptr(), x.numel(), s, y.data_ptr())
+                   for d in self.mats for (x, s, y) in ((d.q16, d.scales[0], d.qq), (d.k16, d.scales[1], d.kq))]
+        v_jobs = [_QuantJobC(d.v16.data_ptr(), d.v16.numel(), d.scales[2], d.vq.data_ptr()) for d in self.mats]
+        self.qk_jobs = _struct_tensor(qk_jobs, self.device)
+        self.v_jobs = _struct_tensor(v_jobs, self.device)
+        self.max_count = max((d.L * HEAD_DIM for d in self.mats), default=0)
+        self.graph = None
+
+    def _desc(self, d: _Dev) -> _AttnMatrixC:
+        ptr = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
+        return _AttnMatrixC(d.L, d.m_vec, d.row_offsets.data_ptr(), d.col_indices.data_ptr(), d.qq.data_ptr(),
+                            d.kq.data_ptr(), d.vq.data_ptr(), d.alpha, d.o_scale, d.o.data_ptr(), ptr(d.o_int),
+                            ptr(d.p_out), ptr(d.s_out))
+
+    # -- fused path ---------------------------------------------------------
+    def quantize(self, stream=None) -> None:
+        lib, st = _lib.lib(), _stream(stream)
+        n = len(self.mats)
+        _lib.check(lib.mc_attn_quantize_batched(_dptr(self.qk_jobs), 2 * n, self.qk_bits, self.rounding,
+                                                self.max_count, st), "mc_attn_quantize_batched")
+        _lib.check(lib.mc_attn_quantize_batched(_dptr(self.v_jobs), n, 8, self.rounding, self.max_count, st),
+                   "mc_attn_quantize_batched")
+
+    def attend(self, stream=None, grid_waves: int = 0) -> None:
+        _lib.check(_lib.lib().mc_attn_forward(_dptr(self.desc), _dptr(self.items), self.n_items, self.qk_bits,
+                                              self.V, self.p_scale, self.p_mode, grid_waves, _stream(stream)),
+                   "mc_attn_forward")
+
+    def forward(self, stream=None, grid_waves: int = 0) -> list[torch.Tensor]:
+        self.quantize(stream)
+        self.attend(stream, grid_waves)
+        return [d.o for d in self.mats]
+
+    def capture(self, stream: torch.cuda.Stream, grid_waves: int = 0) -> torch.cuda.CUDAGraph:
+        """Captures quantize + fused attention into a hipGraph on ``stream``."""
+        with torch.cuda.stream(stream):
+            self.forward(stream, grid_waves)      # warm-up outside the graph
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream):
+            self.forward(stream, grid_waves)
+        self.graph = g
+        return g
+
+    def replay(self, stream: torch.cuda.Stream) -> None:
+        with torch.cuda.stream(stream):
+            self.graph.replay()
+
+    # -- plain one-op-per-launch path -------------------------------------
+    def forward_plain(self, stream=None) -> list[torch.Tensor]:
+        lib, st = _lib.lib(), _stream(stream)
+        V = self.V
+        for d in self.mats:
+            w = d.work
+            if not w:
+                w["s"] = torch.zeros(max(d.aligned_num * V, 16), dtype=torch.int32, device=self.device)
+                w["p"] = torch.zeros(max(d.aligned_num * V, 16), dtype=torch.uint8, device=self.device)
+                w["inv_l"] = torch.empty(max(d.m_vec * V, 1), dtype=torch.float32, device=self.device)
+                w["o_int"] = torch.empty((d.L, HEAD_DIM), dtype=torch.int32, device=self.device)
+                for name, (x, s, y, b) in {"jq": (d.q16, d.scales[0], d.qq, self.qk_bits),
+                                           "jk": (d.k16, d.scales[1], d.kq, self.qk_bits),
+                                           "jv": (d.v16, d.scales[2], d.vq, 8)}.items():
+                    w[name] = (_struct_tensor([_QuantJobC(x.data_ptr(), x.numel(), s, y.data_ptr())], self.device),
+                               b)
+            for name in ("jq", "jk", "jv"):
+                job, b = w[name]
+                _lib.check(lib.mc_attn_quantize_batched(_dptr(job), 1, b, self.rounding, d.L * HEAD_DIM, st),
+                           "quantize")
+            if d.m_vec == 0:
+                continue
+            _lib.check(lib.mc_sddmm_run(self.qk_bits, SIGNED, 1, d.m_vec, HEAD_DIM, d.L, _dptr(d.row_indices),
+                                        _dptr(d.row_offsets), _dptr(d.col_indices), _dptr(d.qq), _dptr(d.kq),
+                                        _dptr(w["s"]), V, st), "sddmm")
+            _lib.check(lib.mc_attn_softmax_requant(d.m_vec, V, _dptr(d.row_indices), _dptr(d.row_offsets),
+                                                   _dptr(w["s"]), d.alpha, self.p_scale, self.p_mode,
+                                                   _dptr(w["p"]), _dptr(w["inv_l"]), st), "softmax")
+            _lib.check(lib.mc_spmm_run(8, 8, SIGNED, -1, d.m_vec, V, HEAD_DIM, d.L, _dptr(d.row_indices),
+                                       _dptr(d.row_offsets), _dptr(d.col_indices), _dptr(w["p"]), _dptr(d.vq),
+                                       _dptr(w["o_int"]), st), "spmm")
+            row_scale = _dptr(w["inv_l"]) if self.p_mode == P_ROW_SCALED else None
+            _lib.check(lib.mc_attn_dequant(_dptr(w["o_int"]), d.L, HEAD_DIM, d.o_scale, row_scale, _dptr(d.o), st),
+                       "dequant")
+        return [d.o for d in self.mats]
+
+    def hbm_bytes(self) -> int:
+        total = 0
+        for d in self.mats:
+            for t in (d.row_offsets, d.col_indices, d.row_indices, d.q16, d.k16, d.v16, d.qq, d.kq, d.vq, d.o):
+                total += t.numel() * t.element_size()
+        return total + self.desc.numel() + self.items.numel() * 4
+
+
+# ---------------------------------------------------------------------------
+# Reference-named op mirrors (src/cuda/quantization.cpp, q_softmax.cpp,
+# deq_sddmm.cpp, deq_spmm.cpp): same roles, device tensors in, tensors out.
+# ---------------------------------------------------------------------------
+
+def bquantization(x: torch.Tensor, bits: int, scale: float, rounding: int = ROUND_TRUNC,
+                  stream=None) -> torch.Tensor:
+    """fp16 [..., L, D] -> int32-packed [..., L, D*bits/32] (trunc(clamp(x*scale)) by default)."""
+    x = x.contiguous()
+    out = torch.empty(x.shape[:-1] + (x.shape[-1] * bits // 32,), dtype=torch.int32, device=x.device)
+    job = _struct_tensor([_QuantJobC(x.data_ptr(), x.numel(), float(scale), out.data_ptr())], x.device)
+    _lib.check(_lib.lib().mc_attn_quantize_batched(_dptr(job), 1, bits, rounding, x.numel(), _stream(stream)),
+               "bquantization")
+    return out
+
+
+def q_bcsr_softmax(row_indices, row_offsets, s_vals, m_vec: int, vec_length: int, alpha: float, p_scale: float,
+                   p_mode: int = P_REFERENCE, aligned_num: int | None = None, stream=None):
+    """int32 logits [slot][V] -> int8 P in the SpMM A layout (+ 1/sum per row)."""
+    n = aligned_num if aligned_num is not None else s_vals.numel() // vec_length
+    p = torch.zeros(max(n * vec_length, 16), dtype=torch.uint8, device=s_vals.device)
+    inv_l = torch.empty(max(m_vec * vec_length, 1), dtype=torch.float32, device=s_vals.device)
+    _lib.check(_lib.lib().mc_attn_softmax_requant(m_vec, vec_length, _dptr(row_indices), _dptr(row_offsets),
+                                                  _dptr(s_vals), alpha, p_scale, p_mode, _dptr(p), _dptr(inv_l),
+                                                  _stream(stream)), "q_bcsr_softmax")
+    return p, inv_l

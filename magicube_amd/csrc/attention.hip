@@ -1,0 +1,418 @@
+// Quantized sparse attention on gfx950: quantize, softmax + requantize,
+// dequantize, and the fused batched kernel (one wave per (matrix, vector row)
+// work item, persistent over a longest-first work list), plus the C ABI.
+#include <hip/hip_fp16.h>
+
+#include "attention_common.hpp"
+#include "mc_internal.hpp"
+#include "sddmm_kernel.hpp"
+
+namespace mc {
+
+// Device view of mc_attn_matrix (same member order and types).
+struct AttnMatrix {
+    int L;
+    int m_vec;
+    const int *row_offsets;
+    const int *col_indices;
+    const unsigned char *q;
+    const unsigned char *k;
+    const unsigned char *v;
+    float alpha;
+    float o_scale;
+    float *o;
+    int *o_int;
+    unsigned char *p_out;
+    int *s_out;
+};
+static_assert(sizeof(AttnMatrix) == sizeof(mc_attn_matrix), "descriptor layout");
+
+// ---------------------------------------------------------------------------
+// quantize: q = trunc(clamp(float(x) * scale, lo, hi)), or rint() for nearest
+// (end2end_eval/sparse_transformer_magicube/src/cuda/quantization_kernel.cu:46-74
+// for 8 bits, :13-43 for 4 bits; 4-bit lanes packed 8 per int32, lane 0 low)
+// ---------------------------------------------------------------------------
+struct QuantJob {
+    const __half *x;
+    long long count;
+    float scale;
+    void *out;
+};
+static_assert(sizeof(QuantJob) == sizeof(mc_quant_job), "job layout");
+
+template <int BITS>
+__global__ __launch_bounds__(256) void quantize_kernel(const QuantJob *jobs, int njobs, int rounding) {
+    const QuantJob jb = jobs[blockIdx.y];
+    const float lo = BITS == 8 ? -128.f : -8.f, hi = BITS == 8 ? 127.f : 7.f;
+    // each thread converts 8 consecutive values
+    const long long groups = (jb.count + 7) / 8;
+    for (long long gi = (long long)blockIdx.x * 256 + threadIdx.x; gi < groups; gi += (long long)gridDim.x * 256) {
+        const long long base = gi * 8;
+        int q[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float f = base + e < jb.count ? __half2float(jb.x[base + e]) * jb.scale : 0.f;
+            f = fminf(fmaxf(f, lo), hi);
+            q[e] = rounding == 0 ? (int)f : (int)rintf(f);
+        }
+        if constexpr (BITS == 8) {
+            unsigned char *o = static_cast<unsigned char *>(jb.out) + base;
+            if (base + 8 <= jb.count) {
+                u32 w0 = (u32)(q[0] & 255) | (u32)(q[1] & 255) << 8 | (u32)(q[2] & 255) << 16 | (u32)(q[3] & 255) << 24;
+                u32 w1 = (u32)(q[4] & 255) | (u32)(q[5] & 255) << 8 | (u32)(q[6] & 255) << 16 | (u32)(q[7] & 255) << 24;
+                *reinterpret_cast<v2i *>(o) = v2i{(int)w0, (int)w1};
+            } else {
+                for (int e = 0; base + e < jb.count; ++e) o[e] = (unsigned char)q[e];
+            }
+        } else {
+            u32 w = 0;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) w |= (u32)(q[e] & 15) << (4 * e);
+            static_cast<u32 *>(jb.out)[gi] = w;
+        }
+    }
+    (void)njobs;
+}
+
+// max |x| over a tensor, combined with atomicMax on the float bit pattern
+// (non-negative floats order like their bits); *out must start at 0.
+__global__ __launch_bounds__(256) void absmax_kernel(const __half *x, long long count, unsigned *out) {
+    float m = 0.f;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < count; i += (long long)gridDim.x * 256)
+        m = fmaxf(m, fabsf(__half2float(x[i])));
+    for (int s = 32; s > 0; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
+    __shared__ float part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float b = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+        atomicMax(out, __float_as_uint(b));
+    }
+}
+
+// dequantize: out[r][c] = (float(in[r][c]) * scale) * row_scale[r]  (row_scale optional)
+__global__ __launch_bounds__(256) void dequant_kernel(const int *in, long long count, int cols, float scale,
+                                                      const float *row_scale, float *out) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < count; i += (long long)gridDim.x * 256) {
+        float f = (float)in[i] * scale;
+        if (row_scale) f = f * row_scale[i / cols];
+        out[i] = f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// softmax + requantize (q_softmax_kernel.cu:180-271): one wave per vector
+// row.  S is int32 vector-major [slot][V] (the SDDMM output); P is written
+// in the SpMM A layout: groups of 16 slots, [V][16] bytes per group.
+// ---------------------------------------------------------------------------
+template <int V, int WPB>
+__global__ __launch_bounds__(64 * WPB) void softmax_requant_kernel(int m_vec, const int *row_indices,
+                                                                   const int *row_offsets, const int *s_vals,
+                                                                   float alpha, float p_scale, int p_mode,
+                                                                   unsigned char *p_out, float *row_inv_l) {
+    constexpr int HOME = V < 4 ? V : 4;
+    __shared__ __attribute__((aligned(16))) unsigned char tiles[WPB][16 * 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row_slot = blockIdx.x * WPB + wave;
+    if (row_slot >= m_vec) return;
+    const int c = lane & 15, g = lane >> 4;
+    const bool home = g < (V + 3) / 4;
+    const int i_row = row_indices ? row_indices[row_slot] : row_slot;
+    const int start = row_offsets[2 * i_row], end = row_offsets[2 * i_row + 1];
+    const int nch = (end - start + 63) >> 6;
+
+    auto load_chunk = [&](int ch, float (&s)[4][4]) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int slot = start + 64 * ch + 16 * t + c;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int v = 4 * g + i;
+                s[t][i] = (home && i < HOME && slot < end) ? (float)s_vals[(size_t)slot * V + v] * alpha : -INFINITY;
+            }
+        }
+    };
+
+    float m[HOME], l[HOME];
+#pragma unroll
+    for (int i = 0; i < HOME; ++i) { m[i] = -INFINITY; l[i] = 0.f; }
+    for (int ch = 0; ch < nch; ++ch) {
+        float s[4][4];
+        load_chunk(ch, s);
+        softmax_chunk_update<HOME>(s, m, l);
+    }
+    float inv_l[HOME];
+#pragma unroll
+    for (int i = 0; i < HOME; ++i) inv_l[i] = l[i] > 0.f ? 1.0f / l[i] : 0.f;
+    if (row_inv_l && home && c == 0) {
+#pragma unroll
+        for (int i = 0; i < HOME; ++i) row_inv_l[(size_t)i_row * V + 4 * g + i] = inv_l[i];
+    }
+    // padded slots of the row (up to the next multiple of 16) get P = 0
+    const int aend = start + ((end - start + 15) & ~15);
+    const int nch_w = (aend - start + 63) >> 6;
+    for (int ch = 0; ch < nch_w; ++ch) {
+        float s[4][4];
+        load_chunk(ch, s);
+        int pq[4][4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                pq[t][i] = i < HOME ? quantize_prob(s[t][i], m[i < HOME ? i : 0], inv_l[i < HOME ? i : 0], p_scale, p_mode) : 0;
+        const v4i a = p_to_a_layout<HOME>(tiles[wave], pq, lane, home);
+        const int slot0 = start + 64 * ch + 16 * g;        // first slot of this lane's 16-slot group
+        if (c < V && slot0 < aend)
+            *reinterpret_cast<v4i *>(p_out + (size_t)(slot0 >> 4) * 16 * V + 16 * c) = a;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused batched attention: persistent waves walk a work list of (matrix,
+// vector row) items.  Pass 1 gathers K rows, computes the int32 logits with
+// MFMA and keeps the online softmax statistics; pass 2 recomputes the exact
+// logits, quantizes P, moves it to the A layout through LDS, gathers V rows
+// (4x4 byte transposes) and accumulates O = P V with MFMA.
+// ---------------------------------------------------------------------------
+template <int V, int QKB, int WPB>
+__global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *mats, const int2 *items, int n_items,
+                                                             float p_scale, int p_mode) {
+    constexpr int HOME = V < 4 ? V : 4;
+    constexpr int QROW = kHeadDim * QKB / 8;          // bytes per Q/K row
+    __shared__ __attribute__((aligned(16))) unsigned char tiles[WPB][16 * 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const bool home = g < (V + 3) / 4;
+    const int nwaves = gridDim.x * WPB;
+
+    for (int it = blockIdx.x * WPB + wave; it < n_items; it += nwaves) {
+        const int2 item = items[it];
+        const AttnMatrix M = mats[item.x];
+        const int i_row = item.y;
+        const int start = M.row_offsets[2 * i_row], end = M.row_offsets[2 * i_row + 1];
+        const int aend = start + ((end - start + 15) & ~15);
+        const int nch = (end - start + 63) >> 6;
+        const int nch_w = (aend - start + 63) >> 6;
+        const u32 kv_bytes = (u32)M.L * QROW;
+        const __amdgpu_buffer_rsrc_t rk = make_rsrc(M.k, kv_bytes);
+        const __amdgpu_buffer_rsrc_t rv = make_rsrc(M.v, (u32)M.L * kHeadDim);
+        const __amdgpu_buffer_rsrc_t rq = make_rsrc(M.q, kv_bytes);
+
+        // SDDMM A operand: Q rows of this vector row (rows >= V read out of range -> 0)
+        v4i qop;
+        {
+            const u32 off = c < V ? (u32)(i_row * V + c) * QROW + (u32)(16 * g * QKB / 8) : 0x80000000u;
+            qop = load_k16<QKB>(rq, off);
+            if constexpr (QKB == 4) qop = expand_nibbles(qop, true, 0u);
+        }
+
+        // logits of one 64-slot chunk in the MFMA C layout, scaled; masked -> -inf
+        auto chunk_logits = [&](int ch, float (&s)[4][4], int (&si)[4][4], int &my_idx) {
+            const int slot_l = start + 64 * ch + lane;                     // this lane's slot for index sharing
+            my_idx = slot_l < end ? M.col_indices[slot_l] : -1;
+            v4i acc[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int idx = __shfl(my_idx, 16 * t + c, 64);
+                const u32 off = idx >= 0 ? (u32)idx * QROW + (u32)(16 * g * QKB / 8) : 0x80000000u;
+                v4i kop = load_k16<QKB>(rk, off);
+                if constexpr (QKB == 4) kop = expand_nibbles(kop, true, 0u);
+                acc[t] = mfma_i8(qop, kop, v4i{0, 0, 0, 0});
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int slot = start + 64 * ch + 16 * t + c;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    si[t][i] = acc[t][i];
+                    s[t][i] = (home && i < HOME && slot < end) ? (float)acc[t][i] * M.alpha : -INFINITY;
+                }
+            }
+        };
+
+        float m[HOME], l[HOME];
+#pragma unroll
+        for (int i = 0; i < HOME; ++i) { m[i] = -INFINITY; l[i] = 0.f; }
+        for (int ch = 0; ch < nch; ++ch) {
+            float s[4][4];
+            int si[4][4];
+            int my_idx;
+            chunk_logits(ch, s, si, my_idx);
+            softmax_chunk_update<HOME>(s, m, l);
+            if (M.s_out && home) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int slot = start + 64 * ch + 16 * t + c;
+#pragma unroll
+                    for (int i = 0; i < HOME; ++i)
+                        if (slot < end) M.s_out[(size_t)slot * V + 4 * g + i] = si[t][i];
+                }
+            }
+        }
+        float inv_l[HOME];
+#pragma unroll
+        for (int i = 0; i < HOME; ++i) inv_l[i] = l[i] > 0.f ? 1.0f / l[i] : 0.f;
+
+        v4i oacc[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) oacc[b] = v4i{0, 0, 0, 0};
+        for (int ch = 0; ch < nch_w; ++ch) {
+            float s[4][4];
+            int si[4][4];
+            int my_idx;
+            chunk_logits(ch, s, si, my_idx);
+            int pq[4][4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    pq[t][i] = i < HOME ? quantize_prob(s[t][i], m[i < HOME ? i : 0], inv_l[i < HOME ? i : 0], p_scale, p_mode) : 0;
+            const v4i pop = p_to_a_layout<HOME>(tiles[wave], pq, lane, home);
+            if (M.p_out) {
+                const int slot0 = start + 64 * ch + 16 * g;
+                if (c < V && slot0 < aend)
+                    *reinterpret_cast<v4i *>(M.p_out + (size_t)(slot0 >> 4) * 16 * V + 16 * c) = pop;
+            }
+            // B operand: V rows of slots 16g + p (p = 0..15), dword c (columns 4c..4c+3)
+            u32 raw[16];
+#pragma unroll
+            for (int p = 0; p < 16; ++p) {
+                const int idx = __shfl(my_idx, 16 * g + p, 64);
+                const u32 off = idx >= 0 ? (u32)idx * kHeadDim + 4u * c : 0x80000000u;
+                raw[p] = buf_load_b32(rv, off);
+            }
+            // after the 4x4 byte transposes, tr[q][b] holds column 4c + b of rows 4q..4q+3
+            u32 tr[4][4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) transpose4x4(raw[4 * q], raw[4 * q + 1], raw[4 * q + 2], raw[4 * q + 3], tr[q]);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const v4i bop = v4i{(int)tr[0][b], (int)tr[1][b], (int)tr[2][b], (int)tr[3][b]};
+                oacc[b] = mfma_i8(pop, bop, oacc[b]);
+            }
+        }
+        // rows 4g+i, columns 4c + b
+        if (home) {
+#pragma unroll
+            for (int i = 0; i < HOME; ++i) {
+                const size_t orow = (size_t)(i_row * V + 4 * g + i) * kHeadDim + 4 * c;
+                if (M.o_int)
+                    *reinterpret_cast<v4i *>(M.o_int + orow) = v4i{oacc[0][i], oacc[1][i], oacc[2][i], oacc[3][i]};
+                // same float operations as dequant_kernel with/without a row scale
+                const float rs = p_mode == kPModeReference ? 1.f : inv_l[i];
+                float4 of;
+                of.x = ((float)oacc[0][i] * M.o_scale) * rs;
+                of.y = ((float)oacc[1][i] * M.o_scale) * rs;
+                of.z = ((float)oacc[2][i] * M.o_scale) * rs;
+                of.w = ((float)oacc[3][i] * M.o_scale) * rs;
+                *reinterpret_cast<float4 *>(M.o + orow) = of;
+            }
+        }
+    }
+}
+
+}  // namespace mc
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+namespace {
+
+inline unsigned grid_for(long long work, long long per_block) {
+    long long b = (work + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > 65535) b = 65535;
+    return (unsigned)b;
+}
+
+}  // namespace
+
+extern "C" {
+
+mc_status mc_attn_quantize_batched(const mc_quant_job *jobs, int njobs, int bits, int rounding,
+                                   long long max_count, hipStream_t stream) {
+    if (bits != 8 && bits != 4) return MC_EUNSUPPORTED;
+    if (rounding != MC_ROUND_TRUNC && rounding != MC_ROUND_NEAREST) return MC_EINVAL_ARG;
+    if (njobs < 0 || max_count < 0) return MC_EINVAL_ARG;
+    if (njobs == 0 || max_count == 0) return MC_OK;
+    if (!jobs || njobs > 65535) return MC_EINVAL_ARG;
+    const unsigned gx = grid_for((max_count + 7) / 8, 256);
+    const auto *dj = reinterpret_cast<const mc::QuantJob *>(jobs);
+    if (bits == 8) hipLaunchKernelGGL(mc::quantize_kernel<8>, dim3(gx, njobs), dim3(256), 0, stream, dj, njobs, rounding);
+    else hipLaunchKernelGGL(mc::quantize_kernel<4>, dim3(gx, njobs), dim3(256), 0, stream, dj, njobs, rounding);
+    return mc::check_launch();
+}
+
+mc_status mc_attn_absmax(const void *x_fp16, long long count, unsigned *out_bits, hipStream_t stream) {
+    if (count < 0 || !out_bits) return MC_EINVAL_ARG;
+    if (count == 0) return MC_OK;
+    if (!x_fp16) return MC_EINVAL_ARG;
+    hipLaunchKernelGGL(mc::absmax_kernel, dim3(grid_for(count, 256 * 16)), dim3(256), 0, stream,
+                       static_cast<const __half *>(x_fp16), count, out_bits);
+    return mc::check_launch();
+}
+
+mc_status mc_attn_dequant(const int *in, int rows, int cols, float scale, const float *row_scale, float *out,
+                          hipStream_t stream) {
+    if (rows < 0 || cols <= 0) return MC_EINVAL_SHAPE;
+    const long long count = (long long)rows * cols;
+    if (count == 0) return MC_OK;
+    if (!in || !out) return MC_EINVAL_ARG;
+    hipLaunchKernelGGL(mc::dequant_kernel, dim3(grid_for(count, 256 * 8)), dim3(256), 0, stream, in, count, cols,
+                       scale, row_scale, out);
+    return mc::check_launch();
+}
+
+mc_status mc_attn_softmax_requant(int m_vec, int vec_length, const int *row_indices, const int *row_offsets,
+                                  const int *s_vals, float alpha, float p_scale, int p_mode, void *p_out,
+                                  float *row_inv_l, hipStream_t stream) {
+    if (p_mode != MC_P_REFERENCE && p_mode != MC_P_ROW_SCALED) return MC_EINVAL_ARG;
+    if (!mc::valid_v(vec_length)) return MC_EINVAL_V;
+    if (m_vec < 0) return MC_EINVAL_SHAPE;
+    if (m_vec == 0) return MC_OK;
+    if (!row_offsets || !s_vals || !p_out) return MC_EINVAL_ARG;
+    constexpr int WPB = 4;
+    const dim3 grid((m_vec + WPB - 1) / WPB), block(64 * WPB);
+    auto *po = static_cast<unsigned char *>(p_out);
+    switch (vec_length) {
+    case 2: hipLaunchKernelGGL((mc::softmax_requant_kernel<2, WPB>), grid, block, 0, stream, m_vec, row_indices, row_offsets, s_vals, alpha, p_scale, p_mode, po, row_inv_l); break;
+    case 4: hipLaunchKernelGGL((mc::softmax_requant_kernel<4, WPB>), grid, block, 0, stream, m_vec, row_indices, row_offsets, s_vals, alpha, p_scale, p_mode, po, row_inv_l); break;
+    default: hipLaunchKernelGGL((mc::softmax_requant_kernel<8, WPB>), grid, block, 0, stream, m_vec, row_indices, row_offsets, s_vals, alpha, p_scale, p_mode, po, row_inv_l); break;
+    }
+    return mc::check_launch();
+}
+
+mc_status mc_attn_forward(const mc_attn_matrix *matrices, const int *items, int n_items, int qk_bits, int vec_length,
+                          float p_scale, int p_mode, int grid_waves, hipStream_t stream) {
+    if (qk_bits != 8 && qk_bits != 4) return MC_EUNSUPPORTED;
+    if (p_mode != MC_P_REFERENCE && p_mode != MC_P_ROW_SCALED) return MC_EINVAL_ARG;
+    if (!mc::valid_v(vec_length)) return MC_EINVAL_V;
+    if (n_items < 0 || grid_waves < 0) return MC_EINVAL_ARG;
+    if (n_items == 0) return MC_OK;
+    if (!matrices || !items) return MC_EINVAL_ARG;
+    constexpr int WPB = 4;
+    long long waves = grid_waves > 0 ? grid_waves : n_items;
+    if (waves > n_items) waves = n_items;
+    const dim3 grid((unsigned)((waves + WPB - 1) / WPB)), block(64 * WPB);
+    const auto *mats = reinterpret_cast<const mc::AttnMatrix *>(matrices);
+    const auto *its = reinterpret_cast<const int2 *>(items);
+#define MC_ATTN_LAUNCH(V_, B_) \
+    hipLaunchKernelGGL((mc::attention_kernel<V_, B_, WPB>), grid, block, 0, stream, mats, its, n_items, p_scale, p_mode)
+    if (qk_bits == 8) {
+        switch (vec_length) {
+        case 2: MC_ATTN_LAUNCH(2, 8); break;
+        case 4: MC_ATTN_LAUNCH(4, 8); break;
+        default: MC_ATTN_LAUNCH(8, 8); break;
+        }
+    } else {
+        switch (vec_length) {
+        case 2: MC_ATTN_LAUNCH(2, 4); break;
+        case 4: MC_ATTN_LAUNCH(4, 4); break;
+        default: MC_ATTN_LAUNCH(8, 4); break;
+        }
+    }
+#undef MC_ATTN_LAUNCH
+    return mc::check_launch();
+}
+
+}  // extern "C"

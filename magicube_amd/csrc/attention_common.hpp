@@ -1,0 +1,103 @@
+// Quantized sparse-attention building blocks shared by the fused batched
+// kernel and the one-op-per-launch kernels, so both produce identical
+// integers (int32 logits, int8 probabilities, int32 outputs).
+//
+// Layer semantics (end2end_eval/sparse_transformer_magicube/spattention.py:
+// 182-216): per (sample, head) matrix with sequence length L and head dim 64,
+//   S = Qq Kq^T at the vector-sparse mask          (int32, exact)
+//   s = float(S) * alpha        alpha = 1 / (scale_q * scale_k * sqrt(64))
+//   reference requantization (q_softmax_kernel.cu:132-147):
+//     P = trunc(min(exp(s - max_row) / sum_row * scale_p, 127))   (int8 >= 0)
+//     O = float(P Vq) * o_scale          o_scale = 1 / (scale_v * scale_p)
+//   row-scaled requantization (full int8 range in every row, rounded):
+//     P = rint(min(exp(s - max_row) * scale_p, 127))
+//     O = (float(P Vq) * o_scale) * (1 / sum_row)
+// Rows are the V matrix rows of one vector row; max/sum run over the row's
+// nonzero slots.
+//
+// Canonical softmax statistics: a row's slots are visited in chunks of 64
+// (four 16-slot MFMA tiles, slot order).  Per chunk: exact max M_c; running
+// max m' = max(m, M_c); chunk sum = (T0 + T1) + (T2 + T3) where each tile sum
+// T_t is a pairwise tree over the tile's 16 slots in slot order; running sum
+// l = l * exp(m - m') + chunk sum.  Both kernels evaluate these statistics
+// on the same register layout (MFMA C layout: lane (c, g) holds rows 4g+i of
+// slot 16t + c) with the same instructions.
+#pragma once
+
+#include "mc_device.hpp"
+
+namespace mc {
+
+constexpr int kHeadDim = 64;
+
+// Butterfly reductions within each 16-lane row of the wave.  Float addition
+// is commutative, so every lane of the row ends with the same bits.
+__device__ __forceinline__ float row16_max(float x) {
+#pragma unroll
+    for (int m = 1; m < 16; m <<= 1) x = fmaxf(x, __shfl_xor(x, m, 64));
+    return x;
+}
+__device__ __forceinline__ float row16_sum(float x) {
+#pragma unroll
+    for (int m = 1; m < 16; m <<= 1) x = x + __shfl_xor(x, m, 64);
+    return x;
+}
+
+// Online statistics of one chunk for the HOME rows held by this lane.
+// s[t][i]: scaled logit of slot 16t + c for row 4g + i (-inf when masked).
+template <int HOME>
+__device__ __forceinline__ void softmax_chunk_update(const float (&s)[4][4], float (&m)[HOME], float (&l)[HOME]) {
+#pragma unroll
+    for (int i = 0; i < HOME; ++i) {
+        float cm = fmaxf(fmaxf(s[0][i], s[1][i]), fmaxf(s[2][i], s[3][i]));
+        cm = row16_max(cm);
+        const float mn = fmaxf(m[i], cm);        // -inf only while every slot so far is masked
+        float tsum[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) tsum[t] = row16_sum(s[t][i] == -INFINITY ? 0.f : __expf(s[t][i] - mn));
+        const float csum = (tsum[0] + tsum[1]) + (tsum[2] + tsum[3]);
+        const float scale_old = (m[i] == -INFINITY) ? 0.f : __expf(m[i] - mn);
+        l[i] = l[i] * scale_old + csum;
+        m[i] = mn;
+    }
+}
+
+constexpr int kPModeReference = 0;
+constexpr int kPModeRowScaled = 1;
+
+// Quantized probability of one logit given the final row statistics.
+__device__ __forceinline__ int quantize_prob(float s, float m, float inv_l, float p_scale, int p_mode) {
+    if (s == -INFINITY) return 0;
+    const float e = __expf(s - m);
+    if (p_mode == kPModeReference) return (int)fminf(e * inv_l * p_scale, 127.f);   // truncation (p >= 0)
+    return (int)rintf(fminf(e * p_scale, 127.f));
+}
+
+
+// Moves P from the MFMA C layout (lane (c, g): rows 4g+i of slot 16t + c) to
+// the MFMA A layout (lane (r, g): row r, slots 16g..16g+15) through a per-wave
+// 16 x 64 byte LDS tile laid out [row][slot].
+template <int HOME>
+__device__ __forceinline__ v4i p_to_a_layout(unsigned char *tile, const int (&pq)[4][4], int lane, bool home) {
+    const int c = lane & 15, g = lane >> 4;
+    if (home) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int i = 0; i < HOME; ++i) tile[(4 * g + i) * 64 + 16 * t + c] = (unsigned char)pq[t][i];
+    }
+    // LDS operations of one wave complete in order; the fences keep the
+    // compiler from moving the read above the writes (and the next writes
+    // above this read).  Rows >= V are never written and only feed output
+    // rows that are discarded.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const v4i a = *reinterpret_cast<const v4i *>(tile + c * 64 + 16 * g);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return a;
+}
+
+}  // namespace mc

@@ -57,3 +57,21 @@ def random_lanes(count: int, bits: int, signed: bool, seed: int) -> np.ndarray:
     else:
         lo, hi = 0, 1 << bits
     return rng.integers(lo, hi, size=count, dtype=np.int64).astype(np.int32)
+
+
+def attention_mask(L: int, V: int, sparsity: float, seed: int) -> VectorCSR:
+    """L x L vector-sparse mask with V-row blocks: each (block, column) kept
+    with probability 1 - sparsity (seeded Bernoulli, vectorized per block)."""
+    rng = np.random.default_rng(seed)
+    m_vec = L // V
+    keep = rng.random((m_vec, L), dtype=np.float32) < np.float32(1.0 - sparsity)
+    rows, cols = np.nonzero(keep)
+    offsets = np.zeros(m_vec + 1, dtype=np.int64)
+    np.cumsum(np.bincount(rows, minlength=m_vec), out=offsets[1:])
+    return VectorCSR(m_vec, L, V, offsets.astype(np.int32), cols.astype(np.int32))
+
+
+def attention_inputs(L: int, seed: int, d: int = 64):
+    """q, k, v ~ N(0, 1) as float16 arrays of shape [L, d]."""
+    rng = np.random.default_rng(seed)
+    return tuple(rng.standard_normal((L, d), dtype=np.float32).astype(np.float16) for _ in range(3))
