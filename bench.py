@@ -134,13 +134,29 @@ def cpu_baseline(csr, a_packed, rhs, M, K, N, V, budget_s=12.0):
 
 
 def load_traffic(name):
+    """HBM bytes per launch recorded in a profiles/ summary ("hbm_bytes_per_launch: N")."""
     path = os.path.join(HERE, "profiles", name)
     try:
         with open(path) as f:
-            d = json.load(f)
-        return d.get("hbm_bytes_per_launch")
+            for line in f:
+                if line.strip().startswith("hbm_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1].strip())
     except (OSError, ValueError):
-        return None
+        pass
+    return None
+
+
+def calibrate_copy(mib=64, iters=20):
+    """A plain device-to-device copy of known size, for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    x = torch.ones(n, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(iters):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": n * 4, "iters": iters}))
 
 
 def main():
@@ -151,7 +167,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reference-shape", action="store_true")
     ap.add_argument("--config", type=int, default=None, help="explicit SpMM tile config index")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 64 MiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               "(the SpMM shards by vector-row blocks with no exchange; see DESIGN.md)", file=sys.stderr)
@@ -180,7 +201,7 @@ def main():
     tops = 2.0 * nnz * V * N / (ms * 1e-3) / 1e12
     alg = algorithmic_bytes(nnz, csr.m_vec, M, K, N, V)
     achieved = alg / (ms * 1e-3) / 1e9
-    traffic = load_traffic("spmm_l8r8_m4096_n512_v8_s090_pmc.json")
+    traffic = load_traffic("spmm_l8r8_m4096_n512_v8_s090_pmc.md")
 
     result = {
         "metric": "SpMM/SDDMM int8 TOPS & roofline %; sparse-attn req/s with p50/p99 latency",
