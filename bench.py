@@ -133,6 +133,127 @@ def cpu_baseline(csr, a_packed, rhs, M, K, N, V, budget_s=12.0):
     }
 
 
+def dense_attention_fp64_torch(q, k, v, mask, V):
+    """softmax(Q K^T / 8) V in float64 on the GPU over the vector-sparse mask
+    (rows of a vector row with no column give zeros): the FP64 reference the
+    quantized layer's error is stated against."""
+    import torch
+    L = q.shape[0]
+    keep = torch.zeros((L // V, L), dtype=torch.bool, device=q.device)
+    rows = torch.repeat_interleave(torch.arange(L // V, device=q.device),
+                                   torch.from_numpy(np.diff(mask.row_offsets).astype(np.int64)).to(q.device))
+    keep[rows, torch.from_numpy(mask.col_indices.astype(np.int64)).to(q.device)] = True
+    keep = keep.repeat_interleave(V, dim=0)
+    s = (q.double() @ k.double().T) / 8.0
+    s = s.masked_fill(~keep, float("-inf"))
+    p = torch.softmax(s, dim=1).nan_to_num(0.0)
+    return p @ v.double()
+
+
+def attention_bench(steps, warmup, seed=101, samples=8, heads=16, V=8, sparsity=0.9):
+    """Batched quantized sparse attention: samples x heads matrices, L ~ U[512, 4096]
+    (multiple of 64) per sample, one mask per (sample, head); fused persistent
+    launch in a HIP graph vs the one-op-per-launch pipeline."""
+    import torch
+    from magicube_amd import attention, synthetic
+    rng = np.random.default_rng(seed)
+    Ls = [int(x) * 64 for x in rng.integers(512 // 64, 4096 // 64 + 1, size=samples)]
+    mats, masks = [], []
+    for s_i, L in enumerate(Ls):
+        for h in range(heads):
+            sd = seed + 1000 * s_i + h
+            q, k, v = synthetic.attention_inputs(L, sd)
+            mask = synthetic.attention_mask(L, V, sparsity, sd + 7)
+            mats.append(attention.AttentionMatrix(*(torch.from_numpy(x).cuda() for x in (q, k, v)), mask))
+            masks.append(mask)
+    batch = attention.AttentionBatch(mats, qk_bits=8, V=V, p_mode=attention.P_ROW_SCALED, p_scale=127.0)
+    stream = torch.cuda.Stream()
+    nnz = sum(m.nnz_vec for m in masks)
+    ops = 2.0 * 2.0 * nnz * V * attention.HEAD_DIM
+    alg = sum(3 * m.L * 64 * 2 + m.L * 64 * 4 + 4 * m.mask.nnz_vec + 4 * (m.mask.m_vec + 1) for m in mats)
+
+    def fused():
+        batch.forward(stream)
+
+    def plain():
+        batch.forward_plain(stream)
+
+    with torch.cuda.stream(stream):
+        plain()
+    torch.cuda.synchronize()
+    plain_out = [d.o.clone() for d in batch.mats]
+    fused_ms, _ = time_gpu(fused, steps, warmup, stream)
+    identical = all(torch.equal(a.view(torch.int32), d.o.view(torch.int32)) for a, d in zip(plain_out, batch.mats))
+    plain_graph_ms, _ = time_gpu(plain, max(1, steps // 4), 2, stream)
+    # plain pipeline launched eagerly, as its callers do (launch overhead included)
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    eager = []
+    with torch.cuda.stream(stream):
+        for _ in range(5):
+            torch.cuda.synchronize()
+            start.record(stream)
+            plain()
+            end.record(stream)
+            torch.cuda.synchronize()
+            eager.append(start.elapsed_time(end))
+    plain_eager_ms = statistics.median(eager)
+    errs = []
+    for m, d, mask in zip(mats, batch.mats, masks):
+        ref = dense_attention_fp64_torch(m.q, m.k, m.v, mask, V)
+        diff = d.o.double() - ref
+        errs.append((float(diff.abs().max()), float(diff.norm() / ref.norm())))
+    return {
+        "workload": f"attention {samples}x{heads} heads, L~U[512,4096] (x64), d=64, V={V}, s={sparsity}, "
+                    "int8 Q/K/V per-tensor 127/max|x|, row-scaled int8 P",
+        "matrices": len(mats), "seq_lens": Ls, "nnz_vec": nnz,
+        "fused_graph_ms": fused_ms, "plain_graph_ms": plain_graph_ms, "plain_eager_ms": plain_eager_ms,
+        "speedup_vs_plain_eager": plain_eager_ms / fused_ms, "speedup_vs_plain_graph": plain_graph_ms / fused_ms,
+        "fused_tops": ops / (fused_ms * 1e-3) / 1e12, "algorithmic_bytes": alg,
+        "fused_hbm_frac": alg / (fused_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "fused_equals_plain": identical,
+        "error_vs_fp64": {"max_abs": max(e[0] for e in errs), "rel_fro_max": max(e[1] for e in errs),
+                          "rel_fro_mean": statistics.mean(e[1] for e in errs)},
+    }
+
+
+def serving_bench(n_requests=48, seed=202, budget_gib=4.0):
+    """Continuous-batching server vs per-request launches on one request
+    trace (12 heads x 1-4 samples, L ~ U[256, 8192], s ~ U[0.8, 0.98], int8 or
+    int4 Q/K): saturation throughput (every request queued at t=0) and
+    latency under Poisson arrivals at 60% of the batched capacity."""
+    import math
+    import torch
+    from magicube_amd import serving
+    budget = int(budget_gib * (1 << 30))
+    warm = serving.make_requests(4, math.inf, seed + 1, heads=2, seq=(256, 1024))
+    for pol in serving.POLICIES:
+        serving.Server(budget, pol).run(warm)
+        serving.reset(warm)
+    reqs = serving.make_requests(n_requests, math.inf, seed)
+    torch.cuda.synchronize()
+    heads = sum(len(r.matrices) for r in reqs)
+    out = {"workload": "attention requests: 12 heads x 1-4 samples, L~U[256,8192] (x64), s~U[0.8,0.98], "
+                       "int8/int4 Q/K, V=8, row-scaled int8 P",
+           "requests": n_requests, "matrices": heads, "hbm_budget_gib": budget_gib}
+    sat = {}
+    for pol in serving.POLICIES:
+        serving.reset(reqs)
+        sat[pol] = serving.Server(budget, pol).run(reqs)
+    out["saturation"] = sat
+    out["speedup_req_per_s_vs_plain"] = sat["batched"]["req_per_s"] / sat["plain"]["req_per_s"]
+    out["speedup_req_per_s_vs_fused_per_request"] = sat["batched"]["req_per_s"] / sat["fused"]["req_per_s"]
+    rate = 0.6 * sat["batched"]["req_per_s"]
+    gaps = np.random.default_rng(seed + 2).exponential(1.0 / rate, size=n_requests)
+    for r, t in zip(reqs, np.cumsum(gaps) - gaps[0]):
+        r.arrival = float(t)
+    poi = {}
+    for pol in serving.POLICIES:
+        serving.reset(reqs)
+        poi[pol] = serving.Server(budget, pol).run(reqs)
+    out["poisson"] = {"rate_req_per_s": rate, **poi}
+    return out
+
+
 def load_traffic(name):
     """HBM bytes per launch recorded in a profiles/ summary ("hbm_bytes_per_launch: N")."""
     path = os.path.join(HERE, "profiles", name)
@@ -166,6 +287,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reference-shape", action="store_true")
+    ap.add_argument("--no-attention", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--config", type=int, default=None, help="explicit SpMM tile config index")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 64 MiB device copy (counter calibration for rocprofv3 --pmc)")
@@ -242,6 +365,12 @@ def main():
                                      "ms_per_step": rms, "tops": rtops, "a100_published_tops": REFERENCE_A100_TOPS,
                                      "ratio_vs_a100": rtops / REFERENCE_A100_TOPS,
                                      "note": "synthetic pattern at the published DLMC shape"}
+
+    if not args.no_attention:
+        result["attention"] = attention_bench(max(1, args.steps // 10), max(1, args.warmup // 4))
+
+    if not args.no_serving:
+        result["serving"] = serving_bench()
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(csr, a_packed, rhs_np, M, K, N, V)

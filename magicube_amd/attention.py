@@ -18,6 +18,7 @@ at the end of the module.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import math
 from dataclasses import dataclass, field
@@ -49,19 +50,70 @@ class _QuantJobC(ctypes.Structure):
                 ("out", ctypes.c_void_p)]
 
 
+def _bytes_tensor(raw: bytes, device) -> torch.Tensor:
+    """Host bytes -> device uint8 tensor, copied asynchronously from pinned
+    memory (the caching host allocator keeps the staging buffer alive until
+    the copy has run)."""
+    device = torch.device(device)
+    host = torch.frombuffer(bytearray(raw if raw else bytes(16)), dtype=torch.uint8)
+    if device.type == "cuda":
+        host = host.pin_memory()
+    return host.to(device, non_blocking=True)
+
+
+def _pinned(a: np.ndarray) -> torch.Tensor:
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    return t.pin_memory() if torch.cuda.is_available() else t
+
+
 def _struct_tensor(structs, device) -> torch.Tensor:
-    raw = b"".join(bytes(s) for s in structs)
-    return torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
+    return _bytes_tensor(b"".join(bytes(s) for s in structs), device)
+
+
+@dataclass
+class ResidentMask:
+    """A vector-sparse attention mask in the kernels' device format: (start,
+    end) row pairs with every row padded to a multiple of 16 slots, column
+    indices padded with -1, a longest-row-first launch order and the per-row
+    nonzero counts."""
+
+    L: int
+    V: int
+    m_vec: int
+    nnz_vec: int
+    aligned_num: int
+    row_offsets: torch.Tensor   # int32 [2 * m_vec]
+    col_indices: torch.Tensor   # int32 [max(aligned_num, 16)]
+    row_indices: torch.Tensor   # int32 [max(m_vec, 1)]
+    row_nnz: torch.Tensor       # int32 [max(m_vec, 1)]
+    pairs: np.ndarray | None = None   # host copy of row_offsets when built from a host mask
+
+    @classmethod
+    def from_csr(cls, mask: formats.VectorCSR, device) -> "ResidentMask":
+        pairs, an = formats.align_rows(mask.row_offsets, 16)
+        cols = formats.align_cols(mask.row_offsets, mask.col_indices, pairs, an, shuffle=False)
+        order = formats.sorted_rows(mask.row_offsets)
+        nnz_rows = np.diff(mask.row_offsets).astype(np.int32)
+        dev = lambda a, fill: torch.from_numpy(a if a.size else fill).to(device)  # noqa: E731
+        return cls(L=mask.k, V=mask.V, m_vec=mask.m_vec, nnz_vec=mask.nnz_vec, aligned_num=an,
+                   row_offsets=dev(pairs, np.zeros(2, np.int32)),
+                   col_indices=dev(cols, np.full(16, -1, np.int32)),
+                   row_indices=dev(order, np.zeros(1, np.int32)),
+                   row_nnz=dev(nnz_rows, np.zeros(1, np.int32)), pairs=pairs)
+
+    def hbm_bytes(self) -> int:
+        return sum(t.numel() * 4 for t in (self.row_offsets, self.col_indices, self.row_indices, self.row_nnz))
 
 
 @dataclass
 class AttentionMatrix:
-    """One (sample, head) problem: fp16 q, k, v of shape [L, 64] and a V-row mask."""
+    """One (sample, head) problem: fp16 q, k, v of shape [L, 64] and a V-row
+    mask (host :class:`formats.VectorCSR` or device :class:`ResidentMask`)."""
 
     q: torch.Tensor
     k: torch.Tensor
     v: torch.Tensor
-    mask: formats.VectorCSR
+    mask: formats.VectorCSR | ResidentMask
 
     @property
     def L(self) -> int:
@@ -74,10 +126,11 @@ class _Dev:
     m_vec: int
     nnz: int
     aligned_num: int
-    pairs: np.ndarray
+    pairs: np.ndarray | None
     row_offsets: torch.Tensor
     col_indices: torch.Tensor
     row_indices: torch.Tensor
+    row_nnz: torch.Tensor
     q16: torch.Tensor
     k16: torch.Tensor
     v16: torch.Tensor
@@ -88,11 +141,59 @@ class _Dev:
     scales: tuple
     alpha: float
     o_scale: float
+    qk_bits: int
     o_int: torch.Tensor | None = None
     p_out: torch.Tensor | None = None
     s_out: torch.Tensor | None = None
     # plain-path work buffers (allocated on first use)
     work: dict = field(default_factory=dict)
+
+    def desc_bytes(self) -> bytes:
+        ptr = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
+        return bytes(_AttnMatrixC(self.L, self.m_vec, self.row_offsets.data_ptr(), self.col_indices.data_ptr(),
+                                  self.qq.data_ptr(), self.kq.data_ptr(), self.vq.data_ptr(), self.alpha,
+                                  self.o_scale, self.o.data_ptr(), ptr(self.o_int), ptr(self.p_out),
+                                  ptr(self.s_out)))
+
+    def quant_job_bytes(self) -> tuple[bytes, bytes]:
+        """(q and k jobs, v job) of the batched quantize launches."""
+        qk = b"".join(bytes(_QuantJobC(x.data_ptr(), x.numel(), s, y.data_ptr()))
+                      for x, s, y in ((self.q16, self.scales[0], self.qq), (self.k16, self.scales[1], self.kq)))
+        return qk, bytes(_QuantJobC(self.v16.data_ptr(), self.v16.numel(), self.scales[2], self.vq.data_ptr()))
+
+
+def working_set_bytes(L: int, qk_bits: int) -> int:
+    """Device bytes a matrix allocates when it joins a batch: quantized q, k,
+    v and the fp32 output."""
+    qk_row = HEAD_DIM * qk_bits // 8
+    return 2 * max(L * qk_row, 16) + max(L * HEAD_DIM, 16) + L * HEAD_DIM * 4
+
+
+def make_resident(m: AttentionMatrix, scales: tuple, qk_bits: int, V: int, p_scale: float, device,
+                  debug: bool = False) -> _Dev:
+    """Uploads and aligns the mask when it is a host mask, and allocates the
+    matrix's quantized operands and output."""
+    mask = m.mask if isinstance(m.mask, ResidentMask) else ResidentMask.from_csr(m.mask, device)
+    if m.q.shape[1] != HEAD_DIM or mask.V != V or mask.m_vec * V != m.L:
+        raise ValueError("matrix shapes do not match the batch")
+    qk_row = HEAD_DIM * qk_bits // 8
+    sq, sk, sv = (float(x) for x in scales)
+    alpha = float(np.float32(1.0) / (np.float32(sq) * np.float32(sk) * np.float32(math.sqrt(HEAD_DIM))))
+    o_scale = float(np.float32(1.0) / (np.float32(sv) * np.float32(p_scale)))
+    d = _Dev(L=m.L, m_vec=mask.m_vec, nnz=mask.nnz_vec, aligned_num=mask.aligned_num, pairs=mask.pairs,
+             row_offsets=mask.row_offsets, col_indices=mask.col_indices, row_indices=mask.row_indices,
+             row_nnz=mask.row_nnz, q16=m.q.contiguous(), k16=m.k.contiguous(), v16=m.v.contiguous(),
+             qq=torch.empty(max(m.L * qk_row, 16), dtype=torch.uint8, device=device),
+             kq=torch.empty(max(m.L * qk_row, 16), dtype=torch.uint8, device=device),
+             vq=torch.empty(max(m.L * HEAD_DIM, 16), dtype=torch.uint8, device=device),
+             o=torch.empty((m.L, HEAD_DIM), dtype=torch.float32, device=device),
+             scales=(sq, sk, sv), alpha=alpha, o_scale=o_scale, qk_bits=qk_bits)
+    if debug:
+        an = mask.aligned_num
+        d.o_int = torch.empty((m.L, HEAD_DIM), dtype=torch.int32, device=device)
+        d.p_out = torch.zeros(max(an * V, 16), dtype=torch.uint8, device=device)
+        d.s_out = torch.zeros(max(an * V, 16), dtype=torch.int32, device=device)
+    return d
 
 
 def tensor_scales(tensors, stream=None) -> list[float]:
@@ -112,68 +213,71 @@ class AttentionBatch:
 
     def __init__(self, mats: list[AttentionMatrix], qk_bits: int = 8, V: int = 8, p_mode: int = P_ROW_SCALED,
                  p_scale: float = 127.0, scales: list[tuple] | None = None, debug: bool = False,
-                 rounding: int | None = None, device="cuda"):
+                 rounding: int | None = None, device="cuda", resident: list[_Dev] | None = None):
         if qk_bits not in (8, 4):
             raise ValueError("qk_bits must be 8 or 4")
         self.qk_bits, self.V, self.p_mode, self.p_scale = qk_bits, V, p_mode, float(p_scale)
         # the reference truncates (quantization_kernel.cu:65-69); the row-scaled layer rounds
         self.rounding = rounding if rounding is not None else (ROUND_TRUNC if p_mode == P_REFERENCE else ROUND_NEAREST)
         self.device = torch.device(device)
-        if scales is None:
-            flat = tensor_scales([t for m in mats for t in (m.q, m.k, m.v)])
-            scales = [tuple(flat[3 * i:3 * i + 3]) for i in range(len(mats))]
-        self.mats: list[_Dev] = []
-        items = []
-        for e, (m, sc) in enumerate(zip(mats, scales)):
-            if m.q.shape[1] != HEAD_DIM or m.mask.V != V or m.mask.m_vec * V != m.L:
-                raise ValueError("matrix shapes do not match the batch")
-            pairs, an = formats.align_rows(m.mask.row_offsets, 16)
-            cols = formats.align_cols(m.mask.row_offsets, m.mask.col_indices, pairs, an, shuffle=False)
-            order = formats.sorted_rows(m.mask.row_offsets)
-            qk_row = HEAD_DIM * qk_bits // 8
-            sq, sk, sv = (float(x) for x in sc)
-            alpha = float(np.float32(1.0) / (np.float32(sq) * np.float32(sk) * np.float32(math.sqrt(HEAD_DIM))))
-            o_scale = float(np.float32(1.0) / (np.float32(sv) * np.float32(self.p_scale)))
-            d = _Dev(L=m.L, m_vec=m.mask.m_vec, nnz=m.mask.nnz_vec, aligned_num=an, pairs=pairs,
-                     row_offsets=torch.from_numpy(pairs if pairs.size else np.zeros(2, np.int32)).to(self.device),
-                     col_indices=torch.from_numpy(cols if cols.size else np.full(16, -1, np.int32)).to(self.device),
-                     row_indices=torch.from_numpy(order).to(self.device),
-                     q16=m.q.contiguous(), k16=m.k.contiguous(), v16=m.v.contiguous(),
-                     qq=torch.empty(max(m.L * qk_row, 16), dtype=torch.uint8, device=self.device),
-                     kq=torch.empty(max(m.L * qk_row, 16), dtype=torch.uint8, device=self.device),
-                     vq=torch.empty(max(m.L * HEAD_DIM, 16), dtype=torch.uint8, device=self.device),
-                     o=torch.empty((m.L, HEAD_DIM), dtype=torch.float32, device=self.device),
-                     scales=(sq, sk, sv), alpha=alpha, o_scale=o_scale)
-            if debug:
-                d.o_int = torch.empty((m.L, HEAD_DIM), dtype=torch.int32, device=self.device)
-                d.p_out = torch.zeros(max(an * V, 16), dtype=torch.uint8, device=self.device)
-                d.s_out = torch.zeros(max(an * V, 16), dtype=torch.int32, device=self.device)
-            self.mats.append(d)
-            nnz_rows = np.diff(m.mask.row_offsets)
-            for i in range(m.mask.m_vec):
-                items.append((int(nnz_rows[i]), e, i))
-        # longest rows first: the strided persistent walk then balances like LPT
-        items.sort(key=lambda t: -t[0])
-        self.n_items = len(items)
-        it = np.array([(e, i) for _, e, i in items], dtype=np.int32).reshape(-1, 2)
-        self.items = torch.from_numpy(it if it.size else np.zeros((1, 2), np.int32)).to(self.device)
-        self.desc = _struct_tensor([self._desc(d) for d in self.mats], self.device)
-        qk_jobs = [_QuantJobC(x.data_ptr(), x.numel(), s, y.data_ptr())
-                   for d in self.mats for (x, s, y) in ((d.q16, d.scales[0], d.qq), (d.k16, d.scales[1], d.kq))]
-        v_jobs = [_QuantJobC(d.v16.data_ptr(), d.v16.numel(), d.scales[2], d.vq.data_ptr()) for d in self.mats]
-        self.qk_jobs = _struct_tensor(qk_jobs, self.device)
-        self.v_jobs = _struct_tensor(v_jobs, self.device)
-        self.max_count = max((d.L * HEAD_DIM for d in self.mats), default=0)
+        if resident is not None:
+            if any(d.qk_bits != qk_bits for d in resident):
+                raise ValueError("resident matrices were quantized for another width")
+            self.mats = list(resident)
+        else:
+            if scales is None:
+                flat = tensor_scales([t for m in mats for t in (m.q, m.k, m.v)])
+                scales = [tuple(flat[3 * i:3 * i + 3]) for i in range(len(mats))]
+            self.mats = [make_resident(m, sc, qk_bits, V, self.p_scale, self.device, debug)
+                         for m, sc in zip(mats, scales)]
+        self._assemble()
         self.graph = None
 
-    def _desc(self, d: _Dev) -> _AttnMatrixC:
-        ptr = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
-        return _AttnMatrixC(d.L, d.m_vec, d.row_offsets.data_ptr(), d.col_indices.data_ptr(), d.qq.data_ptr(),
-                            d.kq.data_ptr(), d.vq.data_ptr(), d.alpha, d.o_scale, d.o.data_ptr(), ptr(d.o_int),
-                            ptr(d.p_out), ptr(d.s_out))
+    @classmethod
+    def from_resident(cls, devs: list[_Dev], **kw) -> "AttentionBatch":
+        """A batch over matrices whose operands are already allocated (their
+        quantization scales, width and output buffers are kept)."""
+        if not devs:
+            raise ValueError("empty batch")
+        return cls([], qk_bits=devs[0].qk_bits, resident=devs, **kw)
+
+    def _assemble(self) -> None:
+        """Work list and descriptors.  Items are (matrix, vector row) pairs in
+        longest-row-first order over the whole batch (sorted on the device),
+        so the strided persistent walk balances like LPT scheduling."""
+        dev = self.device
+        self.n_items = sum(d.m_vec for d in self.mats)
+        if self.n_items:
+            with_rows = [(e, d) for e, d in enumerate(self.mats) if d.m_vec]
+            m_vecs = np.array([d.m_vec for _, d in with_rows], dtype=np.int64)
+            e_idx = np.repeat(np.array([e for e, _ in with_rows], dtype=np.int32), m_vecs)
+            row = (np.arange(self.n_items, dtype=np.int64) - np.repeat(np.cumsum(m_vecs) - m_vecs, m_vecs))
+            host = _pinned(np.stack([e_idx, row.astype(np.int32)], axis=1))
+            unsorted = host.to(dev, non_blocking=True)
+            nnz = torch.cat([d.row_nnz[:d.m_vec] for _, d in with_rows])
+            order = torch.argsort(nnz, descending=True, stable=True)
+            self.items = unsorted[order].contiguous()
+        else:
+            self.items = torch.zeros((1, 2), dtype=torch.int32, device=dev)
+        self.desc = _bytes_tensor(b"".join(d.desc_bytes() for d in self.mats), dev)
+        jobs = [d.quant_job_bytes() for d in self.mats]
+        self.qk_jobs = _bytes_tensor(b"".join(j[0] for j in jobs), dev)
+        self.v_jobs = _bytes_tensor(b"".join(j[1] for j in jobs), dev)
+        self.max_count = max((d.L * HEAD_DIM for d in self.mats), default=0)
+        # the descriptor copies run on the current stream; launches on any
+        # other stream wait for them
+        self._ready = torch.cuda.Event() if dev.type == "cuda" else None
+        if self._ready is not None:
+            self._ready.record()
+
+    def _wait_ready(self, stream) -> None:
+        if self._ready is None or torch.cuda.is_current_stream_capturing():
+            return
+        (stream if stream is not None else torch.cuda.current_stream()).wait_event(self._ready)
 
     # -- fused path ---------------------------------------------------------
     def quantize(self, stream=None) -> None:
+        self._wait_ready(stream)
         lib, st = _lib.lib(), _stream(stream)
         n = len(self.mats)
         _lib.check(lib.mc_attn_quantize_batched(_dptr(self.qk_jobs), 2 * n, self.qk_bits, self.rounding,
@@ -182,6 +286,7 @@ class AttentionBatch:
                    "mc_attn_quantize_batched")
 
     def attend(self, stream=None, grid_waves: int = 0) -> None:
+        self._wait_ready(stream)
         _lib.check(_lib.lib().mc_attn_forward(_dptr(self.desc), _dptr(self.items), self.n_items, self.qk_bits,
                                               self.V, self.p_scale, self.p_mode, grid_waves, _stream(stream)),
                    "mc_attn_forward")
@@ -207,21 +312,29 @@ class AttentionBatch:
             self.graph.replay()
 
     # -- plain one-op-per-launch path -------------------------------------
+    def _plain_buffers(self, d: _Dev, stream) -> dict:
+        """Work buffers of the one-op-per-launch pipeline, created on the
+        launch stream so their job-descriptor copies precede the launches."""
+        w, V = d.work, self.V
+        if w:
+            return w
+        with torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext():
+            w["s"] = torch.zeros(max(d.aligned_num * V, 16), dtype=torch.int32, device=self.device)
+            w["p"] = torch.zeros(max(d.aligned_num * V, 16), dtype=torch.uint8, device=self.device)
+            w["inv_l"] = torch.empty(max(d.m_vec * V, 1), dtype=torch.float32, device=self.device)
+            w["o_int"] = torch.empty((d.L, HEAD_DIM), dtype=torch.int32, device=self.device)
+            for name, (x, s, y, b) in {"jq": (d.q16, d.scales[0], d.qq, self.qk_bits),
+                                       "jk": (d.k16, d.scales[1], d.kq, self.qk_bits),
+                                       "jv": (d.v16, d.scales[2], d.vq, 8)}.items():
+                w[name] = (_struct_tensor([_QuantJobC(x.data_ptr(), x.numel(), s, y.data_ptr())], self.device), b)
+        return w
+
     def forward_plain(self, stream=None) -> list[torch.Tensor]:
+        self._wait_ready(stream)
         lib, st = _lib.lib(), _stream(stream)
         V = self.V
         for d in self.mats:
-            w = d.work
-            if not w:
-                w["s"] = torch.zeros(max(d.aligned_num * V, 16), dtype=torch.int32, device=self.device)
-                w["p"] = torch.zeros(max(d.aligned_num * V, 16), dtype=torch.uint8, device=self.device)
-                w["inv_l"] = torch.empty(max(d.m_vec * V, 1), dtype=torch.float32, device=self.device)
-                w["o_int"] = torch.empty((d.L, HEAD_DIM), dtype=torch.int32, device=self.device)
-                for name, (x, s, y, b) in {"jq": (d.q16, d.scales[0], d.qq, self.qk_bits),
-                                           "jk": (d.k16, d.scales[1], d.kq, self.qk_bits),
-                                           "jv": (d.v16, d.scales[2], d.vq, 8)}.items():
-                    w[name] = (_struct_tensor([_QuantJobC(x.data_ptr(), x.numel(), s, y.data_ptr())], self.device),
-                               b)
+            w = self._plain_buffers(d, stream)
             for name in ("jq", "jk", "jv"):
                 job, b = w[name]
                 _lib.check(lib.mc_attn_quantize_batched(_dptr(job), 1, b, self.rounding, d.L * HEAD_DIM, st),

@@ -75,3 +75,31 @@ def attention_inputs(L: int, seed: int, d: int = 64):
     """q, k, v ~ N(0, 1) as float16 arrays of shape [L, d]."""
     rng = np.random.default_rng(seed)
     return tuple(rng.standard_normal((L, d), dtype=np.float32).astype(np.float16) for _ in range(3))
+
+
+def attention_mask_device(L: int, V: int, sparsity: float, generator, device="cuda"):
+    """Device-generated L x L vector-sparse mask in the kernels' format
+    (a :class:`magicube_amd.attention.ResidentMask`): each (V-row block,
+    column) kept with probability 1 - sparsity, rows padded to 16 slots with
+    -1.  Used where host generation would dominate (long sequences, many
+    requests)."""
+    import torch
+    from .attention import ResidentMask
+    m_vec = L // V
+    keep = torch.rand((m_vec, L), generator=generator, device=device) < (1.0 - sparsity)
+    counts = keep.sum(dim=1)
+    aligned = (counts + 15) // 16 * 16
+    a_start = torch.cumsum(aligned, 0) - aligned
+    u_start = torch.cumsum(counts, 0) - counts
+    rows, cols = keep.nonzero(as_tuple=True)          # row-major, columns ascending
+    total = int(aligned.sum())
+    col_idx = torch.full((max(total, 16),), -1, dtype=torch.int32, device=device)
+    col_idx[a_start[rows] + (torch.arange(rows.numel(), device=device) - u_start[rows])] = cols.to(torch.int32)
+    pairs = torch.stack([a_start, a_start + counts], dim=1).reshape(-1).to(torch.int32)
+    order = torch.argsort(counts, descending=True, stable=True).to(torch.int32)
+    nnz = int(rows.numel())
+    return ResidentMask(L=L, V=V, m_vec=m_vec, nnz_vec=nnz, aligned_num=total,
+                        row_offsets=pairs if m_vec else torch.zeros(2, dtype=torch.int32, device=device),
+                        col_indices=col_idx,
+                        row_indices=order if m_vec else torch.zeros(1, dtype=torch.int32, device=device),
+                        row_nnz=counts.to(torch.int32) if m_vec else torch.zeros(1, dtype=torch.int32, device=device))
