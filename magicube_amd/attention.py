@@ -363,6 +363,141 @@ class AttentionBatch:
         return total + self.desc.numel() + self.items.numel() * 4
 
 
+def _np_dtype(struct) -> np.dtype:
+    """numpy record dtype with the C struct's field offsets and size."""
+    kinds = {ctypes.c_int: np.int32, ctypes.c_float: np.float32, ctypes.c_void_p: np.uint64,
+             ctypes.c_longlong: np.int64}
+    names = [n for n, _ in struct._fields_]
+    return np.dtype({"names": names, "formats": [kinds[t] for _, t in struct._fields_],
+                     "offsets": [getattr(struct, n).offset for n in names], "itemsize": ctypes.sizeof(struct)})
+
+
+_ATTN_DT = _np_dtype(_AttnMatrixC)
+_JOB_DT = _np_dtype(_QuantJobC)
+
+
+def _align256(x):
+    return (x + 255) // 256 * 256
+
+
+class ResidentBlock:
+    """Matrices of one Q/K width (typically one request) whose quantized
+    operands and fp32 outputs share ONE device arena, with their launch
+    descriptors, quantize jobs and work-list items built once, vectorized,
+    when the block is created.  Blocks are concatenated into a launch by
+    :class:`BlockBatch` without per-matrix host work."""
+
+    def __init__(self, mats: list[AttentionMatrix], scales: list[tuple], qk_bits: int, V: int, p_scale: float,
+                 device="cuda"):
+        if qk_bits not in (8, 4):
+            raise ValueError("qk_bits must be 8 or 4")
+        dev = torch.device(device)
+        self.qk_bits, self.V, self.n = qk_bits, V, len(mats)
+        masks = [m.mask if isinstance(m.mask, ResidentMask) else ResidentMask.from_csr(m.mask, dev) for m in mats]
+        ins = [(m.q.contiguous(), m.k.contiguous(), m.v.contiguous()) for m in mats]
+        for m, mk, (q, _, _) in zip(mats, masks, ins):
+            if q.shape[1] != HEAD_DIM or mk.V != V or mk.m_vec * V != m.L:
+                raise ValueError("matrix shapes do not match the block")
+        self._inputs, self._masks = ins, masks            # kept alive with the block
+        n = self.n
+        Ls = np.array([m.L for m in mats], dtype=np.int64)
+        qk_sz = np.maximum(Ls * (HEAD_DIM * qk_bits // 8), 16)
+        v_sz = np.maximum(Ls * HEAD_DIM, 16)
+        o_sz = Ls * HEAD_DIM * 4
+        per = 2 * _align256(qk_sz) + _align256(v_sz) + _align256(o_sz)
+        off = np.cumsum(per) - per
+        self.arena_bytes = int(per.sum())
+        self.arena = torch.empty(max(self.arena_bytes, 256), dtype=torch.uint8, device=dev)
+        base = np.uint64(self.arena.data_ptr())
+        q_off = off
+        k_off = q_off + _align256(qk_sz)
+        v_off = k_off + _align256(qk_sz)
+        self._o_off, self._Ls = v_off + _align256(v_sz), Ls
+        sc = np.asarray(scales, dtype=np.float32).reshape(n, 3)
+        alpha = np.float32(1.0) / (sc[:, 0] * sc[:, 1] * np.float32(math.sqrt(HEAD_DIM)))
+        o_scale = np.float32(1.0) / (sc[:, 2] * np.float32(p_scale))
+        d = np.zeros(n, dtype=_ATTN_DT)
+        d["L"] = Ls
+        d["m_vec"] = [mk.m_vec for mk in masks]
+        d["row_offsets"] = [mk.row_offsets.data_ptr() for mk in masks]
+        d["col_indices"] = [mk.col_indices.data_ptr() for mk in masks]
+        d["q"], d["k"], d["v"] = base + q_off.astype(np.uint64), base + k_off.astype(np.uint64), \
+            base + v_off.astype(np.uint64)
+        d["alpha"], d["o_scale"] = alpha, o_scale
+        d["o"] = base + self._o_off.astype(np.uint64)
+        self.desc = d
+        x_ptr = np.array([[t.data_ptr() for t in trio] for trio in ins], dtype=np.uint64).reshape(n, 3)
+        qk = np.zeros((n, 2), dtype=_JOB_DT)
+        qk["x"] = x_ptr[:, :2]
+        qk["count"] = (Ls * HEAD_DIM)[:, None]
+        qk["scale"] = sc[:, :2]
+        qk["out"] = np.stack([d["q"], d["k"]], axis=1)
+        self.qk_jobs = qk.reshape(-1)
+        vj = np.zeros(n, dtype=_JOB_DT)
+        vj["x"], vj["count"], vj["scale"], vj["out"] = x_ptr[:, 2], Ls * HEAD_DIM, sc[:, 2], d["v"]
+        self.v_jobs = vj
+        self.max_count = int(Ls.max()) * HEAD_DIM if n else 0
+        m_vecs = d["m_vec"].astype(np.int64)
+        self.n_items = int(m_vecs.sum())
+        if self.n_items:
+            e = np.repeat(np.arange(n, dtype=np.int32), m_vecs)
+            row = (np.arange(self.n_items, dtype=np.int64) - np.repeat(np.cumsum(m_vecs) - m_vecs, m_vecs))
+            self.items_local = _pinned(np.stack([e, row.astype(np.int32)], axis=1)).to(dev, non_blocking=True)
+            self.row_nnz = torch.cat([mk.row_nnz[:mk.m_vec] for mk in masks if mk.m_vec])
+        else:
+            self.items_local = self.row_nnz = None
+
+    def outputs(self) -> list[torch.Tensor]:
+        """fp32 [L, 64] output view of every matrix."""
+        o = self.arena
+        return [o[int(a):int(a) + int(L) * HEAD_DIM * 4].view(torch.float32).view(int(L), HEAD_DIM)
+                for a, L in zip(self._o_off, self._Ls)]
+
+
+class BlockBatch:
+    """One fused launch (batched quantize + persistent attention) over the
+    union of resident blocks of one Q/K width: descriptors concatenated,
+    items merged longest-row-first on the device."""
+
+    def __init__(self, blocks: list[ResidentBlock], p_mode: int = P_ROW_SCALED, p_scale: float = 127.0,
+                 rounding: int | None = None, device="cuda"):
+        if not blocks or any(b.qk_bits != blocks[0].qk_bits or b.V != blocks[0].V for b in blocks):
+            raise ValueError("a block batch needs blocks of one width and V")
+        dev = torch.device(device)
+        self.qk_bits, self.V, self.p_mode, self.p_scale = blocks[0].qk_bits, blocks[0].V, p_mode, float(p_scale)
+        self.rounding = rounding if rounding is not None else (ROUND_TRUNC if p_mode == P_REFERENCE else ROUND_NEAREST)
+        self.blocks = blocks
+        self.n_mats = sum(b.n for b in blocks)
+        # raw bytes per block: np.concatenate would repack the padded C
+        # record layout (numpy drops the padding of structured dtypes when it
+        # promotes them)
+        self.desc = _bytes_tensor(b"".join(b.desc.tobytes() for b in blocks), dev)
+        self.qk_jobs = _bytes_tensor(b"".join(b.qk_jobs.tobytes() for b in blocks), dev)
+        self.v_jobs = _bytes_tensor(b"".join(b.v_jobs.tobytes() for b in blocks), dev)
+        self.max_count = max(b.max_count for b in blocks)
+        with_items = [(e0, b) for e0, b in zip(np.cumsum([0] + [b.n for b in blocks[:-1]]), blocks) if b.n_items]
+        self.n_items = sum(b.n_items for _, b in with_items)
+        if self.n_items:
+            items = torch.cat([b.items_local for _, b in with_items])
+            if len(with_items) > 1:
+                shift = _pinned(np.repeat(np.array([e0 for e0, _ in with_items], dtype=np.int32),
+                                          [b.n_items for _, b in with_items])).to(dev, non_blocking=True)
+                items[:, 0] += shift
+            nnz = torch.cat([b.row_nnz for _, b in with_items])
+            self.items = items[torch.argsort(nnz, descending=True, stable=True)].contiguous()
+        else:
+            self.items = torch.zeros((1, 2), dtype=torch.int32, device=dev)
+
+    def forward(self, stream=None, grid_waves: int = 0) -> None:
+        lib, st = _lib.lib(), _stream(stream)
+        _lib.check(lib.mc_attn_quantize_batched(_dptr(self.qk_jobs), 2 * self.n_mats, self.qk_bits, self.rounding,
+                                                self.max_count, st), "mc_attn_quantize_batched")
+        _lib.check(lib.mc_attn_quantize_batched(_dptr(self.v_jobs), self.n_mats, 8, self.rounding, self.max_count,
+                                                st), "mc_attn_quantize_batched")
+        _lib.check(lib.mc_attn_forward(_dptr(self.desc), _dptr(self.items), self.n_items, self.qk_bits, self.V,
+                                       self.p_scale, self.p_mode, grid_waves, st), "mc_attn_forward")
+
+
 # ---------------------------------------------------------------------------
 # Reference-named op mirrors (src/cuda/quantization.cpp, q_softmax.cpp,
 # deq_sddmm.cpp, deq_spmm.cpp): same roles, device tensors in, tensors out.

@@ -50,7 +50,12 @@ class Request:
     admitted: float = math.nan
     launched: float = math.nan
     done: float = math.nan
-    outputs: list | None = None         # fp32 [L, 64] per matrix (when kept)
+    outputs: list | None = None         # fp32 [L, 64] per matrix (plain policy)
+    block: object = None                # attention.ResidentBlock holding the outputs (fused policies)
+
+    def result(self) -> list:
+        """fp32 [L, 64] output of every matrix (valid once the request is done)."""
+        return self.block.outputs() if self.block is not None else self.outputs
 
     @property
     def latency(self) -> float:
@@ -99,43 +104,81 @@ class Server:
 
     def __init__(self, hbm_budget: int, policy: str = "batched", V: int = 8, p_mode: int = attention.P_ROW_SCALED,
                  p_scale: float = 127.0, max_inflight: int = 2, n_streams: int = 2, grid_waves: int = 0,
-                 keep_outputs: bool = False, device="cuda"):
+                 keep_outputs: bool = False, max_step_items: int | None = None, device="cuda"):
         if policy not in POLICIES:
             raise ValueError(f"policy must be one of {POLICIES}")
         self.budget, self.policy, self.V = int(hbm_budget), policy, V
         self.p_mode, self.p_scale = p_mode, float(p_scale)
         self.max_inflight, self.grid_waves, self.keep_outputs = max_inflight, grid_waves, keep_outputs
         self.device = torch.device(device)
+        self.max_step_items = max_step_items
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(n_streams)]
+        self.prep_stream = torch.cuda.Stream(device=self.device)
+        self.prep_event = torch.cuda.Event()
         self._next_stream = 0
         self.steps = 0
         self.peak_used = 0
 
-    # -- one step -----------------------------------------------------------
+    # -- admission and one step -------------------------------------------
+    def _admit(self, r: Request, now: float) -> None:
+        """Reserves the request's working set; for the fused policies also
+        allocates its arena and builds its descriptors (on the preparation
+        stream, which every step waits for)."""
+        r.admitted = now
+        if self.policy != "plain":
+            with torch.cuda.stream(self.prep_stream):
+                r.block = attention.ResidentBlock(r.matrices, r.scales, r.qk_bits, self.V, self.p_scale, self.device)
+            self.prep_event.record(self.prep_stream)
+
     def _launch(self, reqs: list[Request], now: float) -> _Step:
         stream = self.streams[self._next_stream]
         self._next_stream = (self._next_stream + 1) % len(self.streams)
         keep = []
         with torch.cuda.stream(stream):
-            groups: dict[int, list] = {}
             for r in reqs:
                 r.launched = now
-                devs = [attention.make_resident(m, sc, r.qk_bits, self.V, self.p_scale, self.device)
-                        for m, sc in zip(r.matrices, r.scales)]
-                r.outputs = [d.o for d in devs]
-                groups.setdefault(r.qk_bits, []).extend(devs)
-            for bits, devs in groups.items():
-                batch = attention.AttentionBatch.from_resident(devs, V=self.V, p_mode=self.p_mode,
-                                                               p_scale=self.p_scale, device=self.device)
-                if self.policy == "plain":
+            if self.policy == "plain":
+                groups: dict[int, list] = {}
+                for r in reqs:
+                    devs = [attention.make_resident(m, sc, r.qk_bits, self.V, self.p_scale, self.device)
+                            for m, sc in zip(r.matrices, r.scales)]
+                    r.outputs = [d.o for d in devs]
+                    groups.setdefault(r.qk_bits, []).extend(devs)
+                for devs in groups.values():
+                    batch = attention.AttentionBatch.from_resident(devs, V=self.V, p_mode=self.p_mode,
+                                                                   p_scale=self.p_scale, device=self.device)
                     batch.forward_plain(stream)
-                else:
+                    keep.append(batch)
+            else:
+                stream.wait_event(self.prep_event)
+                blocks: dict[int, list] = {}
+                for r in reqs:
+                    blocks.setdefault(r.qk_bits, []).append(r.block)
+                for blks in blocks.values():
+                    batch = attention.BlockBatch(blks, p_mode=self.p_mode, p_scale=self.p_scale, device=self.device)
                     batch.forward(stream, self.grid_waves)
-                keep.append(batch)
+                    keep.append(batch)
             ev = torch.cuda.Event()
             ev.record(stream)
         self.steps += 1
         return _Step(reqs, ev, keep)
+
+    def _take(self, ready: list[Request]) -> tuple[list[Request], list[Request]]:
+        """The requests of the next step: one for the per-request policies;
+        for the batched policy every ready request, up to ``max_step_items``
+        work items (always at least one request)."""
+        if self.policy != "batched":
+            return ready[:1], ready[1:]
+        if self.max_step_items is None:
+            return ready, []
+        n, items = 0, 0
+        while n < len(ready):
+            it = ready[n].block.n_items
+            if n and items + it > self.max_step_items:
+                break
+            items += it
+            n += 1
+        return ready[:n], ready[n:]
 
     # -- the loop -------------------------------------------------------------
     def run(self, requests: list[Request]) -> dict:
@@ -160,17 +203,18 @@ class Server:
                     r.done = t
                     used -= r.ws_bytes
                     if not self.keep_outputs:
-                        r.outputs = None
-            while waiting and used + waiting[0].ws_bytes <= self.budget:
-                r = waiting.popleft()
-                r.admitted = now
-                used += r.ws_bytes
-                ready.append(r)
-            self.peak_used = max(self.peak_used, used)
+                        r.outputs = r.block = None
             if ready and len(inflight) < self.max_inflight:
-                take = ready if self.policy == "batched" else ready[:1]
-                ready = [] if self.policy == "batched" else ready[1:]
+                take, ready = self._take(ready)
                 inflight.append(self._launch(take, now))
+            # admit one request per pass, so a free stream is never kept
+            # waiting behind a long admission burst
+            if waiting and used + waiting[0].ws_bytes <= self.budget:
+                r = waiting.popleft()
+                used += r.ws_bytes
+                self._admit(r, now)
+                ready.append(r)
+                self.peak_used = max(self.peak_used, used)
         span = max(r.done for r in requests) - min(r.arrival for r in requests) if requests else 0.0
         lat = sorted(r.latency for r in requests)
         return {
@@ -199,4 +243,4 @@ def reset(requests: list[Request]) -> None:
     """Clears the timing and outputs of a trace so it can be served again."""
     for r in requests:
         r.admitted = r.launched = r.done = math.nan
-        r.outputs = None
+        r.outputs = r.block = None

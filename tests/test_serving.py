@@ -16,3 +16,100 @@ def test_working_set_counts_quantized_operands_and_output():
     assert attention.working_set_bytes(1024, 8) == 1024 * 64 * 7
     # int4 q, k halve
     assert attention.working_set_bytes(1024, 4) == 1024 * 64 * 6
+
+
+def _cpu_mats():
+    import torch
+    from magicube_amd import synthetic
+    mats = []
+    for n, L in enumerate((256, 520, 64)):
+        q, k, v = (torch.from_numpy(x) for x in synthetic.attention_inputs(L, n))
+        mats.append(attention.AttentionMatrix(q, k, v, synthetic.attention_mask(L, 8, 0.9, n)))
+    scales = [(127.0 / 3.1, 127.0 / 2.9, 127.0 / 4.2 + n) for n in range(3)]
+    return mats, scales
+
+
+def test_resident_block_descriptors_match_per_matrix_path():
+    """The vectorized block builds the same launch descriptors (fields and
+    operand sizes) as the per-matrix path."""
+    import numpy as np
+    mats, scales = _cpu_mats()
+    for bits in (8, 4):
+        blk = attention.ResidentBlock(mats, scales, bits, 8, 127.0, device="cpu")
+        devs = [attention.make_resident(m, sc, bits, 8, 127.0, "cpu") for m, sc in zip(mats, scales)]
+        ref = np.frombuffer(b"".join(d.desc_bytes() for d in devs), dtype=attention._ATTN_DT)
+        for f in ("L", "m_vec", "row_offsets", "col_indices", "alpha", "o_scale", "o_int", "p_out", "s_out"):
+            if f in ("row_offsets", "col_indices"):
+                continue            # the block uploads its own copy of a host mask
+            assert np.array_equal(blk.desc[f], ref[f]), f
+        # operand regions: q, k sized L*64*bits/8, v L*64, o L*64*4, inside the arena, disjoint
+        base = blk.arena.data_ptr()
+        spans = []
+        for i, d in enumerate(devs):
+            for f, size in (("q", d.qq.numel()), ("k", d.kq.numel()), ("v", d.vq.numel()), ("o", d.o.numel() * 4)):
+                start = int(blk.desc[f][i]) - base
+                spans.append((start, start + size))
+        spans.sort()
+        assert spans[0][0] >= 0 and spans[-1][1] <= blk.arena.numel()
+        assert all(a[1] <= b[0] for a, b in zip(spans, spans[1:]))
+        jobs = np.frombuffer(b"".join(d.quant_job_bytes()[0] for d in devs), dtype=attention._JOB_DT)
+        assert np.array_equal(blk.qk_jobs["count"], jobs["count"])
+        assert np.array_equal(blk.qk_jobs["scale"], jobs["scale"])
+        assert np.array_equal(blk.qk_jobs["x"], jobs["x"])
+        # work items: every (matrix, vector row) exactly once
+        items = blk.items_local.numpy()
+        exp = [(e, i) for e, d in enumerate(devs) for i in range(d.m_vec)]
+        assert sorted(map(tuple, items.tolist())) == exp
+        outs = blk.outputs()
+        assert [tuple(o.shape) for o in outs] == [(m.L, 64) for m in mats]
+        assert [o.data_ptr() for o in outs] == [int(x) for x in blk.desc["o"]]
+
+
+def test_block_batch_device_bytes_decode_to_valid_operands():
+    """Decodes the descriptor and quantize-job buffers a block batch uploads,
+    in the C record layout, and checks every pointer the kernels follow lands
+    inside a live tensor of the right size (regression: concatenating padded
+    records with numpy once repacked the 32-byte jobs to 28 bytes)."""
+    import numpy as np
+    import torch
+    from magicube_amd import synthetic
+    gen = torch.Generator().manual_seed(3)
+    blocks, objs = [], []
+    for rid, (bits, Ls) in enumerate(((8, (256, 512)), (8, (320,)), (4, (448, 64)))):
+        mats = []
+        for L in Ls:
+            q, k, v = (torch.randn((L, 64), generator=gen).half() for _ in range(3))
+            mats.append(attention.AttentionMatrix(q, k, v, synthetic.attention_mask_device(L, 8, 0.85, gen, "cpu")))
+        blk = attention.ResidentBlock(mats, [(30.0, 31.0, 29.0 + rid)] * len(mats), bits, 8, 127.0, device="cpu")
+        blocks.append((bits, blk))
+        objs += [blk.arena] + [t for m in mats for t in (m.q, m.k, m.v, m.mask.row_offsets, m.mask.col_indices)]
+    regs = [(t.data_ptr(), t.data_ptr() + t.numel() * t.element_size()) for t in objs]
+    by_ptr = {t.data_ptr(): t for t in objs}
+
+    def inside(p, n):
+        return any(a <= p and p + n <= b for a, b in regs)
+
+    for bits in (8, 4):
+        bb = attention.BlockBatch([b for w, b in blocks if w == bits], device="cpu")
+        raw = lambda t: t.numpy().tobytes()  # noqa: E731
+        assert bb.desc.numel() == 88 * bb.n_mats and bb.qk_jobs.numel() == 64 * bb.n_mats
+        assert bb.v_jobs.numel() == 32 * bb.n_mats
+        desc = np.frombuffer(raw(bb.desc), dtype=attention._ATTN_DT)
+        for j in np.frombuffer(raw(bb.qk_jobs), dtype=attention._JOB_DT).tolist():
+            x, cnt, _, out = j
+            assert 0 < cnt <= bb.max_count and inside(x, 2 * cnt) and inside(out, cnt * bits // 8)
+        for x, cnt, _, out in np.frombuffer(raw(bb.v_jobs), dtype=attention._JOB_DT).tolist():
+            assert inside(x, 2 * cnt) and inside(out, cnt)
+        items = bb.items.numpy().tolist()
+        assert len(set(map(tuple, items))) == bb.n_items == sum(int(d["m_vec"]) for d in desc)
+        for e, row in items:
+            d = desc[e]
+            L = int(d["L"])
+            assert 0 <= row < int(d["m_vec"])
+            ro, ci = by_ptr[int(d["row_offsets"])], by_ptr[int(d["col_indices"])]
+            s0, s1 = int(ro[2 * row]), int(ro[2 * row + 1])
+            assert 0 <= s0 <= s1 <= ci.numel() and s0 % 16 == 0
+            assert s1 == s0 or 0 <= int(ci[s0:s1].min()) and int(ci[s0:s1].max()) < L
+            assert inside(int(d["o"]) + row * 8 * 256, 8 * 256)
+            for f, size in (("q", L * 8 * bits), ("k", L * 8 * bits), ("v", L * 64)):
+                assert inside(int(d[f]), size)

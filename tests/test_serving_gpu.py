@@ -28,19 +28,19 @@ def test_alone_equals_mixed_and_plain():
     torch.cuda.synchronize()
     assert stats["peak_working_set_bytes"] <= budget
     assert stats["steps"] < len(reqs), "the batched server should merge requests"
-    mixed = {r.rid: _bits(r.outputs) for r in reqs}
+    mixed = {r.rid: _bits(r.result()) for r in reqs}
     for r in reqs:
         serving.reset([r])
         serving.Server(budget, "batched", keep_outputs=True).run([r])
         torch.cuda.synchronize()
-        alone = _bits(r.outputs)
+        alone = _bits(r.result())
         for a, b in zip(alone, mixed[r.rid]):
             assert np.array_equal(a, b), f"request {r.rid} differs alone vs mixed"
     serving.reset(reqs)
     serving.Server(budget, "plain", keep_outputs=True).run(reqs)
     torch.cuda.synchronize()
     for r in reqs:
-        for a, b in zip(_bits(r.outputs), mixed[r.rid]):
+        for a, b in zip(_bits(r.result()), mixed[r.rid]):
             assert np.array_equal(a, b), f"request {r.rid} differs fused vs plain"
 
 
