@@ -70,7 +70,8 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
             continue
         out = os.path.join(HERE, exe)
         cmd = [HIPCC, "-O3", "-std=c++17", "-x", "c++", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
-               path, "-o", out, f"-L{HERE}", "-lmagicube_hip", f"-Wl,-rpath,{HERE}", "-lamdhip64"]
+               path, "-o", out, f"-L{HERE}", "-lmagicube_hip", "-Wl,-rpath,$ORIGIN", f"-L{ROCM}/lib",
+               "-lamdhip64"]
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:
             raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

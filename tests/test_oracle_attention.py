@@ -1,0 +1,120 @@
+"""Pins the attention oracles (oracle/attention.py) on the CPU: quantization
+against a scalar loop, the FP64 dense attention against torch's float64
+softmax, exact integers by brute force, closed-form single-column rows, the
+quantized layer's distance from FP64, and planted mistakes that the FP64
+comparison must expose."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from magicube_amd import formats, synthetic
+from oracle import attention as oatt
+
+
+def _scalar_quant(x, scale, bits, rounding):
+    lo, hi = (-128.0, 127.0) if bits == 8 else (-8.0, 7.0)
+    f = float(np.float32(np.float32(x) * np.float32(scale)))
+    f = min(max(f, lo), hi)
+    return int(math.trunc(f)) if rounding == 0 else int(round(f))   # round(): half to even
+
+
+@pytest.mark.parametrize("bits", [8, 4])
+@pytest.mark.parametrize("rounding", [0, 1])
+def test_quantize_ref_matches_scalar_loop(bits, rounding):
+    edges = [0.0, 0.5, -0.5, 1.5, -1.5, 2.5, -2.5, 0.49, 126.6, 127.5, -128.7, 300.0, -300.0, 7.5, -8.5, 6.9]
+    rng = np.random.default_rng(1)
+    x = np.concatenate([np.array(edges), rng.standard_normal(200) * 4]).astype(np.float16)
+    for scale in (1.0, 36.0 / 9.0, 0.73):
+        got = oatt.quantize_ref(x, scale, bits, rounding)
+        exp = [_scalar_quant(v, scale, bits, rounding) for v in x]
+        assert got.tolist() == exp
+
+
+def _tiny(L=64, V=8, sparsity=0.7, seed=0, empty_row=True):
+    q, k, v = synthetic.attention_inputs(L, seed)
+    mask = synthetic.attention_mask(L, V, sparsity, seed + 1)
+    if empty_row:                                   # vector row 1 keeps no column
+        ro, ci = mask.row_offsets.copy(), mask.col_indices
+        n1 = ro[2] - ro[1]
+        ci = np.concatenate([ci[:ro[1]], ci[ro[2]:]])
+        ro[2:] -= n1
+        mask = formats.VectorCSR(mask.m_vec, L, V, ro, ci)
+    return q, k, v, mask
+
+
+def test_dense_fp64_matches_torch_masked_softmax():
+    q, k, v, mask = _tiny()
+    L, V = q.shape[0], mask.V
+    keep = np.zeros((L, L), bool)
+    for i in range(mask.m_vec):
+        keep[i * V:(i + 1) * V, mask.col_indices[mask.row_offsets[i]:mask.row_offsets[i + 1]]] = True
+    s = torch.from_numpy(q).double() @ torch.from_numpy(k).double().T / 8.0
+    p = torch.softmax(s.masked_fill(~torch.from_numpy(keep), float("-inf")), dim=1).nan_to_num(0.0)
+    exp = (p @ torch.from_numpy(v).double()).numpy()
+    got = oatt.dense_attention_fp64(q, k, v, mask)
+    assert np.allclose(got, exp, rtol=0, atol=1e-12)
+    assert (got[8:16] == 0).all()
+
+
+def test_single_column_rows_are_closed_form():
+    L, V = 32, 8
+    q, k, v = synthetic.attention_inputs(L, 3)
+    cols = np.array([5, 17, 0, 31], np.int32)
+    mask = formats.VectorCSR(4, L, V, np.arange(5, dtype=np.int32), cols)
+    o = oatt.dense_attention_fp64(q, k, v, mask)
+    for i, c in enumerate(cols):                    # softmax over one column = 1
+        assert np.array_equal(o[i * V:(i + 1) * V], np.repeat(v[c:c + 1].astype(np.float64), V, 0))
+    scales = (30.0, 31.0, 32.0)
+    r = oatt.quantized_attention_ref(q, k, v, mask, scales, p_mode=1, p_scale=127.0)
+    for i, c in enumerate(cols):
+        assert (r["P"][i] == 127).all()
+        assert np.array_equal(r["o_int"][i * V:(i + 1) * V], 127 * np.repeat(r["vq"][c:c + 1], V, 0))
+    r0 = oatt.quantized_attention_ref(q, k, v, mask, scales, p_mode=0, p_scale=32.0)
+    assert all((P == 32).all() for P in r0["P"])
+
+
+def test_quantized_integers_by_brute_force():
+    q, k, v, mask = _tiny(L=32, sparsity=0.5, seed=4)
+    V = mask.V
+    r = oatt.quantized_attention_ref(q, k, v, mask, (20.0, 21.0, 22.0), qk_bits=8)
+    qq, kq, vq = r["qq"].tolist(), r["kq"].tolist(), r["vq"].tolist()
+    for i in range(mask.m_vec):
+        cols = mask.col_indices[mask.row_offsets[i]:mask.row_offsets[i + 1]].tolist()
+        for rr in range(V):
+            for j, c in enumerate(cols):
+                assert r["S"][i][rr, j] == sum(a * b for a, b in zip(qq[i * V + rr], kq[c]))
+            for d in range(64):
+                exp = sum(int(r["P"][i][rr, j]) * vq[c][d] for j, c in enumerate(cols))
+                assert r["o_int"][i * V + rr, d] == exp
+
+
+def test_quantized_layer_is_close_to_fp64():
+    q, k, v = synthetic.attention_inputs(512, 7)
+    mask = synthetic.attention_mask(512, 8, 0.9, 8)
+    sc = tuple(127.0 / float(np.abs(t.astype(np.float32)).max()) for t in (q, k, v))
+    ref = oatt.dense_attention_fp64(q, k, v, mask)
+    good = oatt.error_vs_fp64(oatt.quantized_attention_ref(q, k, v, mask, sc)["o"], ref)
+    assert good["rel_fro"] < 0.035, good
+
+
+@pytest.mark.parametrize("mistake", ["swapped_qk", "missing_sqrt_d", "shifted_mask", "sign_flip_v"])
+def test_planted_mistakes_move_far_from_fp64(mistake):
+    q, k, v = synthetic.attention_inputs(512, 7)
+    mask = synthetic.attention_mask(512, 8, 0.9, 8)
+    sc = [127.0 / float(np.abs(t.astype(np.float32)).max()) for t in (q, k, v)]
+    ref = oatt.dense_attention_fp64(q, k, v, mask)
+    qq, kk, vv, mm = q, k, v, mask
+    if mistake == "swapped_qk":
+        qq, kk = k, q
+    elif mistake == "missing_sqrt_d":
+        # same quantized integers, alpha 8x too large: 1 / sqrt(d) dropped
+        qq = (q.astype(np.float32) * 8).astype(np.float16)
+        sc[0] /= 8.0
+    elif mistake == "shifted_mask":
+        mm = formats.VectorCSR(mask.m_vec, mask.k, mask.V, mask.row_offsets, (mask.col_indices + 1) % mask.k)
+    elif mistake == "sign_flip_v":
+        vv = (-v.astype(np.float32)).astype(np.float16)
+    bad = oatt.error_vs_fp64(oatt.quantized_attention_ref(qq, kk, vv, mm, tuple(sc))["o"], ref)
+    assert bad["rel_fro"] > 0.2, (mistake, bad)
