@@ -77,18 +77,24 @@ mc_status mc_spmm_l4r4(int m_vec, int vec_length, int n, int k, const int *row_i
                        const int *rhs_matrix, int *output_matrix, mc_signedness sign,
                        hipStream_t stream);
 
-/* Tile configurations of the SpMM kernel (an index into a fixed table):
- * lane_dwords B dwords gathered per lane per row, tiles_per_block column
- * tiles per workgroup, split_k waves sharing one (row, tile) over the
- * nonzero vectors, depth k-blocks of gathers in flight per wave.  No
- * reference counterpart (the reference fixes its tile
- * per template, SpMM/SpMM/src/wmma_spmm.cu:2137-2153); used by the
- * autotuner. */
+/* Tile configurations of the SpMM kernels (an index into a fixed table).
+ * kind 0, gather kernel: lane_dwords B dwords gathered per lane per row,
+ *   tiles_per_block column tiles per workgroup, split_k waves sharing one
+ *   (row, tile) over the nonzero vectors, depth k-blocks of gathers in
+ *   flight per wave.
+ * kind 1, B-stationary kernel (8-bit B; the K rows of a column slice of B
+ *   are held in LDS): lane_dwords B dwords per row of the slice (n must be a
+ *   multiple of 4*lane_dwords and (k+1)*4*lane_dwords bytes must fit in LDS),
+ *   tiles_per_block waves per workgroup, split_k vector rows per workgroup,
+ *   depth k-blocks of indices and A operands in flight per wave.
+ * No reference counterpart (the reference fixes its tile per template,
+ * SpMM/SpMM/src/wmma_spmm.cu:2137-2153); used by the autotuner. */
 typedef struct mc_tile_config {
     int lane_dwords;
     int tiles_per_block;
     int split_k;
     int depth;
+    int kind;
 } mc_tile_config;
 int mc_spmm_num_tile_configs(void);
 mc_status mc_spmm_get_tile_config(int index, mc_tile_config *out);

@@ -25,8 +25,11 @@ int default_config(int lhs_bits, int rhs_bits, int V, int n) {
 }
 
 // Four dwords per lane are built for 8-bit B only (4-bit B would hold 32
-// accumulator tiles per wave).
-bool config_supported(int cfg, int rhs_bits) { return rhs_bits == 8 || kSpmmConfigs[cfg].lane_dwords < 4; }
+// accumulator tiles per wave); the B-stationary kernel is built for 8-bit B.
+bool config_supported(int cfg, int rhs_bits) {
+    if (kSpmmConfigs[cfg].kind == 1) return rhs_bits == 8;
+    return rhs_bits == 8 || kSpmmConfigs[cfg].lane_dwords < 4;
+}
 
 bool pair_supported(int lhs_bits, int rhs_bits) {
     return (lhs_bits == 8 && rhs_bits == 8) || (lhs_bits == 16 && rhs_bits == 8) ||
@@ -50,6 +53,10 @@ mc_status spmm_entry(int lhs_bits, int rhs_bits, mc_signedness sign, int config,
     if (config < -1 || config >= kNumSpmmConfigs) return MC_EINVAL_ARG;
     int cfg = config >= 0 ? config : default_config(lhs_bits, rhs_bits, vec_length, n);
     if (!config_supported(cfg, rhs_bits)) return MC_EUNSUPPORTED;
+    if (kSpmmConfigs[cfg].kind == 1 && !stationary_fits(cfg, n, k)) {
+        if (config >= 0) return MC_EINVAL_SHAPE;
+        cfg = 7;   // the gather kernel takes any k
+    }
     if ((n / cpd) % kSpmmConfigs[cfg].lane_dwords != 0) {
         if (config >= 0) return MC_EINVAL_SHAPE;
         cfg = 4;   // one dword per lane fits every n

@@ -5,15 +5,29 @@
 
 #include "mc_internal.hpp"
 #include "spmm_kernel.hpp"
+#include "spmm_stationary.hpp"
 
 namespace mc {
 
-// Tile configurations: {lane_dwords, tiles_per_block, split_k, depth}.
-constexpr int kNumSpmmConfigs = 8;
+// Tile configurations {lane_dwords, tiles_per_block, split_k, depth, kind}:
+// 0-7 gather kernel; 8-11 B-stationary kernel (8-bit B), for which the fields
+// are {B dwords per slice row, waves per workgroup, vector rows per
+// workgroup, prefetch depth}.
+constexpr int kNumSpmmConfigs = 12;
+constexpr int kFirstStationaryConfig = 8;
 constexpr mc_tile_config kSpmmConfigs[kNumSpmmConfigs] = {
-    {2, 4, 1, 2}, {2, 4, 2, 2}, {4, 2, 2, 2}, {4, 2, 4, 2},
-    {1, 4, 2, 3}, {2, 2, 2, 2}, {4, 1, 4, 2}, {1, 8, 1, 3},
+    {2, 4, 1, 2, 0}, {2, 4, 2, 2, 0}, {4, 2, 2, 2, 0}, {4, 2, 4, 2, 0},
+    {1, 4, 2, 3, 0}, {2, 2, 2, 2, 0}, {4, 1, 4, 2, 0}, {1, 8, 1, 3, 0},
+    {8, 16, 32, 4, 1}, {8, 8, 32, 4, 1}, {16, 16, 32, 4, 1}, {8, 16, 16, 4, 1},
 };
+
+// Whether a B-stationary config fits the shape (slice width divides n, the
+// K rows of the slice fit in LDS).
+inline bool stationary_fits(int cfg, int n, int k) {
+    const mc_tile_config &c = kSpmmConfigs[cfg];
+    const int tb = 4 * c.lane_dwords;
+    return c.kind == 1 && n % tb == 0 && (size_t)(k + 1) * tb <= (size_t)kStatLdsBytes;
+}
 
 template <int V, int AB, int BB, bool SGN, int LW, int TPB, int SK, int D>
 mc_status spmm_launch_one(SpmmArgs a, hipStream_t stream) {
@@ -46,6 +60,18 @@ mc_status spmm_launch_cfg(int cfg, const SpmmArgs &a, hipStream_t s) {
         if constexpr (BB == 8) return spmm_launch_one<V, AB, BB, SGN, 4, 1, 4, 2>(a, s);
         else return MC_EUNSUPPORTED;
     case 7: return spmm_launch_one<V, AB, BB, SGN, 1, 8, 1, 3>(a, s);
+    case 8:
+        if constexpr (BB == 8) return spmm_stationary_launch<V, AB, SGN, 32, 16, 4>(a, 32, s);
+        else return MC_EUNSUPPORTED;
+    case 9:
+        if constexpr (BB == 8) return spmm_stationary_launch<V, AB, SGN, 32, 8, 4>(a, 32, s);
+        else return MC_EUNSUPPORTED;
+    case 10:
+        if constexpr (BB == 8) return spmm_stationary_launch<V, AB, SGN, 64, 16, 4>(a, 32, s);
+        else return MC_EUNSUPPORTED;
+    case 11:
+        if constexpr (BB == 8) return spmm_stationary_launch<V, AB, SGN, 32, 16, 4>(a, 16, s);
+        else return MC_EUNSUPPORTED;
     default: return MC_EINVAL_ARG;
     }
 }
