@@ -169,7 +169,8 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) 
     __shared__ __attribute__((aligned(16))) u32 s_off[kStageSlots];
 
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // wave-uniform in an SGPR, so every loop bound below is scalar
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = lane & 15, g = lane >> 4;
     const int tile_local = wave % TPB;
     const int split = wave / TPB;
@@ -291,17 +292,27 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) 
         u32 ring[DEPTH][16][LW];
         v4i aring[DEPTH][RT];
 #pragma unroll
-        for (int d = 0; d < DEPTH; ++d) issue(split + d * SPLITK, ring[d], aring[d]);
-        for (int base = split; base < cblk; base += DEPTH * SPLITK) {
+        for (int d = 0; d < DEPTH; ++d) {
+            issue(split + d * SPLITK, ring[d], aring[d]);
+            __builtin_amdgcn_sched_barrier(0);      // the loop expects this issue order
+        }
+        // Main loop: every stage of the ring holds a real k-block, no branches
+        // inside, so the wait counts keep DEPTH-1 stages of gathers in flight
+        // across the back edge.  Then the last partial round.
+        int base = split;
+        for (; base + (DEPTH - 1) * SPLITK < cblk; base += DEPTH * SPLITK) {
 #pragma unroll
             for (int d = 0; d < DEPTH; ++d) {
-                const int kb = base + d * SPLITK;
-                if (kb < cblk) {
-                    compute(ring[d], aring[d]);
-                    issue(kb + DEPTH * SPLITK, ring[d], aring[d]);
-                }
+                compute(ring[d], aring[d]);
+                issue(base + d * SPLITK + DEPTH * SPLITK, ring[d], aring[d]);
+                // keep stage d's work here: hoisting the next stages' transposes
+                // above this point would wait for their gathers too
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
+#pragma unroll
+        for (int d = 0; d < DEPTH; ++d)
+            if (base + d * SPLITK < cblk) compute(ring[d], aring[d]);
     }
 
     // ---- epilogue: recombine planes (shifted adds), offset correction ----

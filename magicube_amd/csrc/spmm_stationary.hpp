@@ -50,7 +50,7 @@ __global__ __launch_bounds__(64 * WPB) void spmm_stationary_kernel(SpmmArgs args
     extern __shared__ __attribute__((aligned(16))) unsigned char bslice[];
 
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = lane & 15, g = lane >> 4;
     const int rg = blockIdx.x % row_groups;
     const int tile = blockIdx.x / row_groups;
@@ -141,7 +141,10 @@ __global__ __launch_bounds__(64 * WPB) void spmm_stationary_kernel(SpmmArgs args
 
     Stage ring[DEPTH];
 #pragma unroll
-    for (int d = 0; d < DEPTH; ++d) issue(ring[d]);
+    for (int d = 0; d < DEPTH; ++d) {
+        issue(ring[d]);
+        __builtin_amdgcn_sched_barrier(0);      // the loop expects this issue order
+    }
 
     // ---- the B slice: rows 0..K-1 of columns [tile*TB, tile*TB + TB), then a zero row
     {
@@ -276,16 +279,19 @@ __global__ __launch_bounds__(64 * WPB) void spmm_stationary_kernel(SpmmArgs args
         }
     }
 
-    for (int base = 0; base < total; base += DEPTH) {
+    // branch-free main loop while every stage holds a k-block, then the tail
+    int base = 0;
+    for (; base + DEPTH <= total; base += DEPTH) {
 #pragma unroll
         for (int d = 0; d < DEPTH; ++d) {
-            const int p = base + d;
-            if (p < total) {
-                compute(ring[d]);
-                issue(ring[d]);
-            }
+            compute(ring[d]);
+            issue(ring[d]);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
+#pragma unroll
+    for (int d = 0; d < DEPTH; ++d)
+        if (base + d < total) compute(ring[d]);
 }
 
 // Launch geometry of the stationary kernel for a shape (0 when the shape
