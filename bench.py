@@ -237,6 +237,10 @@ def serving_bench(n_requests=48, seed=202, budget_gib=4.0):
            "requests": n_requests, "matrices": heads, "hbm_budget_gib": budget_gib}
     sat = {}
     for pol in serving.POLICIES:
+        # one untimed pass per policy on the same trace: allocator growth for
+        # the per-request arenas and first launches are not serving cost
+        serving.reset(reqs)
+        serving.Server(budget, pol).run(reqs)
         serving.reset(reqs)
         sat[pol] = serving.Server(budget, pol).run(reqs)
     out["saturation"] = sat

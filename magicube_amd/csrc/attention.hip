@@ -112,13 +112,15 @@ __global__ __launch_bounds__(64 * WPB) void softmax_requant_kernel(int m_vec, co
                                                                    unsigned char *p_out, float *row_inv_l) {
     constexpr int HOME = V < 4 ? V : 4;
     __shared__ __attribute__((aligned(16))) unsigned char tiles[WPB][16 * 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave id in an SGPR keeps work-item indices, descriptors and buffer
+    // resources wave-uniform (no per-lane waterfall around each buffer load)
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int row_slot = blockIdx.x * WPB + wave;
     if (row_slot >= m_vec) return;
     const int c = lane & 15, g = lane >> 4;
     const bool home = g < (V + 3) / 4;
-    const int i_row = row_indices ? row_indices[row_slot] : row_slot;
-    const int start = row_offsets[2 * i_row], end = row_offsets[2 * i_row + 1];
+    const int i_row = uniform(row_indices ? row_indices[row_slot] : row_slot);
+    const int start = uniform(row_offsets[2 * i_row]), end = uniform(row_offsets[2 * i_row + 1]);
     const int nch = (end - start + 63) >> 6;
 
     auto load_chunk = [&](int ch, float (&s)[4][4]) {
@@ -180,16 +182,32 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
     constexpr int HOME = V < 4 ? V : 4;
     constexpr int QROW = kHeadDim * QKB / 8;          // bytes per Q/K row
     __shared__ __attribute__((aligned(16))) unsigned char tiles[WPB][16 * 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave id in an SGPR keeps work-item indices, descriptors and buffer
+    // resources wave-uniform (no per-lane waterfall around each buffer load)
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c = lane & 15, g = lane >> 4;
     const bool home = g < (V + 3) / 4;
     const int nwaves = gridDim.x * WPB;
 
     for (int it = blockIdx.x * WPB + wave; it < n_items; it += nwaves) {
         const int2 item = items[it];
-        const AttnMatrix M = mats[item.x];
-        const int i_row = item.y;
-        const int start = M.row_offsets[2 * i_row], end = M.row_offsets[2 * i_row + 1];
+        const AttnMatrix Mv = mats[uniform(item.x)];
+        AttnMatrix M;
+        M.L = uniform(Mv.L);
+        M.m_vec = uniform(Mv.m_vec);
+        M.row_offsets = uniform(Mv.row_offsets);
+        M.col_indices = uniform(Mv.col_indices);
+        M.q = uniform(Mv.q);
+        M.k = uniform(Mv.k);
+        M.v = uniform(Mv.v);
+        M.alpha = uniform(Mv.alpha);
+        M.o_scale = uniform(Mv.o_scale);
+        M.o = uniform(Mv.o);
+        M.o_int = uniform(Mv.o_int);
+        M.p_out = uniform(Mv.p_out);
+        M.s_out = uniform(Mv.s_out);
+        const int i_row = uniform(item.y);
+        const int start = uniform(M.row_offsets[2 * i_row]), end = uniform(M.row_offsets[2 * i_row + 1]);
         const int aend = start + ((end - start + 15) & ~15);
         const int nch = (end - start + 63) >> 6;
         const int nch_w = (aend - start + 63) >> 6;

@@ -73,6 +73,20 @@ __device__ __forceinline__ v4i buf_load_b128(__amdgpu_buffer_rsrc_t r, u32 off) 
     return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
 }
 
+// Wave-uniform copies of values every lane loaded from the same address.
+// Values in VGPRs cannot feed a buffer resource or a scalar branch without a
+// per-lane waterfall loop; one v_readfirstlane each makes them SGPR-resident.
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uniform(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+template <typename T> __device__ __forceinline__ T *uniform(T *p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return reinterpret_cast<T *>(((unsigned long long)hi << 32) | lo);
+}
+
 // Cross-lane read of lane (self + delta) within the wave (delta multiple of 16).
 __device__ __forceinline__ int lane_down(int x, int delta) {
     const int lane = __lane_id();

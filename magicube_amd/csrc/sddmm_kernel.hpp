@@ -81,11 +81,11 @@ __global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
     __shared__ __attribute__((aligned(16))) u32 s_off[kSddmmStage];
 
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = lane & 15, g = lane >> 4;
-    const int m = args.row_indices ? args.row_indices[blockIdx.x] : blockIdx.x;
-    const int start = args.row_offsets[2 * m];
-    const int end = args.row_offsets[2 * m + 1];
+    const int m = uniform(args.row_indices ? args.row_indices[blockIdx.x] : (int)blockIdx.x);
+    const int start = uniform(args.row_offsets[2 * m]);
+    const int end = uniform(args.row_offsets[2 * m + 1]);
     const int wend = start + ((end - start + 7) & ~7);   // written slots
 
     const u32 row_bytes = (u32)args.k * BITS / 8;
