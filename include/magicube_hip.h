@@ -203,12 +203,18 @@ typedef struct mc_attn_matrix {
 } mc_attn_matrix;
 /* Fused quantized sparse attention over many matrices in one persistent
  * launch.  items is a device array of n_items (matrix index, vector row)
- * int pairs, processed by grid_waves waves (0: one wave per item) in list
- * order with a stride of grid_waves; any order gives the same results.  No
- * reference counterpart: the reference launches each op separately with one
- * mask shared by the batch (spattention.py:189-210). */
+ * int pairs (pairs with a negative matrix index are skipped), processed by
+ * grid_waves waves (0: one wave per item) in list order with a stride of
+ * grid_waves; any order gives the same results.  Item i runs in workgroup
+ * i / 4, and consecutive workgroups go to the 8 XCDs in turn, so an order
+ * that keeps each matrix on one XCD (see the Python package) keeps its K/V
+ * rows in that XCD's L2.
+ * debug != 0 also writes the descriptors' non-null o_int / p_out / s_out
+ * (a slower kernel); with debug == 0 they are ignored.  No reference
+ * counterpart: the reference launches each op separately with one mask
+ * shared by the batch (spattention.py:189-210). */
 mc_status mc_attn_forward(const mc_attn_matrix *matrices, const int *items, int n_items, int qk_bits,
-                          int vec_length, float p_scale, int p_mode, int grid_waves,
+                          int vec_length, float p_scale, int p_mode, int grid_waves, int debug,
                           hipStream_t stream);
 
 /* ---- Host-side format preparation (no device work) ----------------------

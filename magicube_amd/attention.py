@@ -208,6 +208,56 @@ def tensor_scales(tensors, stream=None) -> list[float]:
     return [127.0 / float(a) if a > 0 else 1.0 for a in amax]
 
 
+N_XCD = 8              # MI355X: workgroups are handed to the 8 XCDs in turn
+WAVES_PER_BLOCK = 4    # waves per workgroup of the fused attention launch
+DEFAULT_GRID_WAVES = 8192
+
+
+def xcd_bins(m_vecs, works, n_xcd: int = N_XCD) -> list[np.ndarray]:
+    """Rows of the matrices dealt to each XCD: n_xcd int32 [k, 2] arrays of
+    (matrix, vector row).  Matrices sorted by ``works`` (largest first) go to
+    the XCDs in boustrophedon order (0..7, 7..0, ...), which balances the
+    XCDs' work without a per-matrix loop."""
+    m_vecs = np.asarray(m_vecs, dtype=np.int64)
+    works = np.asarray(works, dtype=np.float64)
+    es = np.argsort(-works, kind="stable")
+    es = es[m_vecs[es] > 0]
+    k = np.arange(es.size)
+    lane = k % n_xcd
+    bins = np.where((k // n_xcd) % 2 == 0, lane, n_xcd - 1 - lane)
+    lists = []
+    for x in range(n_xcd):
+        ex = es[bins == x]
+        mv = m_vecs[ex]
+        e_col = np.repeat(ex, mv).astype(np.int32)
+        r_col = (np.arange(int(mv.sum())) - np.repeat(np.cumsum(mv) - mv, mv)).astype(np.int32)
+        lists.append(np.stack([e_col, r_col], axis=1))
+    return lists
+
+
+def interleave_bins(lists, wpb: int = WAVES_PER_BLOCK) -> np.ndarray:
+    """Work list of the fused launch from per-XCD row lists: the rows of XCD
+    x fill the workgroups b = x, x + n, x + 2n, ... (wpb items each), which
+    the hardware hands to XCD x, so each matrix's K/V rows are read through
+    one XCD's L2.  Slots left over in the shorter lists hold (-1, -1) and are
+    skipped by the kernel."""
+    n_xcd = len(lists)
+    blocks = max((-(-len(li) // wpb) for li in lists), default=0)
+    if blocks == 0:
+        return np.full((1, 2), -1, dtype=np.int32)
+    padded = np.full((n_xcd, blocks * wpb, 2), -1, dtype=np.int32)
+    for x, li in enumerate(lists):
+        padded[x, :len(li)] = li
+    # [xcd][block][wave] -> [block][xcd][wave]: workgroup b * n_xcd + x on XCD x
+    return np.ascontiguousarray(padded.reshape(n_xcd, blocks, wpb, 2).transpose(1, 0, 2, 3)).reshape(-1, 2)
+
+
+def xcd_item_order(m_vecs, works, n_xcd: int = N_XCD, wpb: int = WAVES_PER_BLOCK) -> np.ndarray:
+    """Work list of the fused launch: int32 [n, 2] (matrix, vector row), each
+    matrix's rows on one XCD (:func:`xcd_bins`, :func:`interleave_bins`)."""
+    return interleave_bins(xcd_bins(m_vecs, works, n_xcd), wpb)
+
+
 class AttentionBatch:
     """Device-resident quantized sparse attention over uneven matrices."""
 
@@ -217,6 +267,7 @@ class AttentionBatch:
         if qk_bits not in (8, 4):
             raise ValueError("qk_bits must be 8 or 4")
         self.qk_bits, self.V, self.p_mode, self.p_scale = qk_bits, V, p_mode, float(p_scale)
+        self.debug = bool(debug)
         # the reference truncates (quantization_kernel.cu:65-69); the row-scaled layer rounds
         self.rounding = rounding if rounding is not None else (ROUND_TRUNC if p_mode == P_REFERENCE else ROUND_NEAREST)
         self.device = torch.device(device)
@@ -242,23 +293,11 @@ class AttentionBatch:
         return cls([], qk_bits=devs[0].qk_bits, resident=devs, **kw)
 
     def _assemble(self) -> None:
-        """Work list and descriptors.  Items are (matrix, vector row) pairs in
-        longest-row-first order over the whole batch (sorted on the device),
-        so the strided persistent walk balances like LPT scheduling."""
+        """Work list (XCD-aware order, :func:`xcd_item_order`) and descriptors."""
         dev = self.device
-        self.n_items = sum(d.m_vec for d in self.mats)
-        if self.n_items:
-            with_rows = [(e, d) for e, d in enumerate(self.mats) if d.m_vec]
-            m_vecs = np.array([d.m_vec for _, d in with_rows], dtype=np.int64)
-            e_idx = np.repeat(np.array([e for e, _ in with_rows], dtype=np.int32), m_vecs)
-            row = (np.arange(self.n_items, dtype=np.int64) - np.repeat(np.cumsum(m_vecs) - m_vecs, m_vecs))
-            host = _pinned(np.stack([e_idx, row.astype(np.int32)], axis=1))
-            unsorted = host.to(dev, non_blocking=True)
-            nnz = torch.cat([d.row_nnz[:d.m_vec] for _, d in with_rows])
-            order = torch.argsort(nnz, descending=True, stable=True)
-            self.items = unsorted[order].contiguous()
-        else:
-            self.items = torch.zeros((1, 2), dtype=torch.int32, device=dev)
+        order = xcd_item_order([d.m_vec for d in self.mats], [d.nnz + 16 * d.m_vec for d in self.mats])
+        self.n_items = int(len(order)) if any(d.m_vec for d in self.mats) else 0
+        self.items = _pinned(order).to(dev, non_blocking=True)
         self.desc = _bytes_tensor(b"".join(d.desc_bytes() for d in self.mats), dev)
         jobs = [d.quant_job_bytes() for d in self.mats]
         self.qk_jobs = _bytes_tensor(b"".join(j[0] for j in jobs), dev)
@@ -285,18 +324,19 @@ class AttentionBatch:
         _lib.check(lib.mc_attn_quantize_batched(_dptr(self.v_jobs), n, 8, self.rounding, self.max_count, st),
                    "mc_attn_quantize_batched")
 
-    def attend(self, stream=None, grid_waves: int = 0) -> None:
+    def attend(self, stream=None, grid_waves: int = DEFAULT_GRID_WAVES) -> None:
         self._wait_ready(stream)
         _lib.check(_lib.lib().mc_attn_forward(_dptr(self.desc), _dptr(self.items), self.n_items, self.qk_bits,
-                                              self.V, self.p_scale, self.p_mode, grid_waves, _stream(stream)),
+                                              self.V, self.p_scale, self.p_mode, grid_waves, int(self.debug),
+                                              _stream(stream)),
                    "mc_attn_forward")
 
-    def forward(self, stream=None, grid_waves: int = 0) -> list[torch.Tensor]:
+    def forward(self, stream=None, grid_waves: int = DEFAULT_GRID_WAVES) -> list[torch.Tensor]:
         self.quantize(stream)
         self.attend(stream, grid_waves)
         return [d.o for d in self.mats]
 
-    def capture(self, stream: torch.cuda.Stream, grid_waves: int = 0) -> torch.cuda.CUDAGraph:
+    def capture(self, stream: torch.cuda.Stream, grid_waves: int = DEFAULT_GRID_WAVES) -> torch.cuda.CUDAGraph:
         """Captures quantize + fused attention into a hipGraph on ``stream``."""
         with torch.cuda.stream(stream):
             self.forward(stream, grid_waves)      # warm-up outside the graph
@@ -385,7 +425,7 @@ class ResidentBlock:
     operands and fp32 outputs share ONE device arena, with their launch
     descriptors, quantize jobs and work-list items built once, vectorized,
     when the block is created.  Blocks are concatenated into a launch by
-    :class:`BlockBatch` without per-matrix host work."""
+    :class:`BlockBatch` without per-matrix device work."""
 
     def __init__(self, mats: list[AttentionMatrix], scales: list[tuple], qk_bits: int, V: int, p_scale: float,
                  device="cuda"):
@@ -437,15 +477,11 @@ class ResidentBlock:
         vj["x"], vj["count"], vj["scale"], vj["out"] = x_ptr[:, 2], Ls * HEAD_DIM, sc[:, 2], d["v"]
         self.v_jobs = vj
         self.max_count = int(Ls.max()) * HEAD_DIM if n else 0
+        self.nnz_vec = np.array([mk.nnz_vec for mk in masks], dtype=np.int64)
         m_vecs = d["m_vec"].astype(np.int64)
         self.n_items = int(m_vecs.sum())
-        if self.n_items:
-            e = np.repeat(np.arange(n, dtype=np.int32), m_vecs)
-            row = (np.arange(self.n_items, dtype=np.int64) - np.repeat(np.cumsum(m_vecs) - m_vecs, m_vecs))
-            self.items_local = _pinned(np.stack([e, row.astype(np.int32)], axis=1)).to(dev, non_blocking=True)
-            self.row_nnz = torch.cat([mk.row_nnz[:mk.m_vec] for mk in masks if mk.m_vec])
-        else:
-            self.items_local = self.row_nnz = None
+        # per-XCD row lists with block-local matrix indices, built once
+        self.xcd_lists = xcd_bins(m_vecs, self.nnz_vec + 16 * m_vecs)
 
     def outputs(self) -> list[torch.Tensor]:
         """fp32 [L, 64] output view of every matrix."""
@@ -475,27 +511,29 @@ class BlockBatch:
         self.qk_jobs = _bytes_tensor(b"".join(b.qk_jobs.tobytes() for b in blocks), dev)
         self.v_jobs = _bytes_tensor(b"".join(b.v_jobs.tobytes() for b in blocks), dev)
         self.max_count = max(b.max_count for b in blocks)
-        with_items = [(e0, b) for e0, b in zip(np.cumsum([0] + [b.n for b in blocks[:-1]]), blocks) if b.n_items]
-        self.n_items = sum(b.n_items for _, b in with_items)
-        if self.n_items:
-            items = torch.cat([b.items_local for _, b in with_items])
-            if len(with_items) > 1:
-                shift = _pinned(np.repeat(np.array([e0 for e0, _ in with_items], dtype=np.int32),
-                                          [b.n_items for _, b in with_items])).to(dev, non_blocking=True)
-                items[:, 0] += shift
-            nnz = torch.cat([b.row_nnz for _, b in with_items])
-            self.items = items[torch.argsort(nnz, descending=True, stable=True)].contiguous()
-        else:
-            self.items = torch.zeros((1, 2), dtype=torch.int32, device=dev)
+        # block j's XCD lists rotate by j so the first (largest) bins of the
+        # requests spread over the XCDs; matrix indices shift to the batch's
+        base = np.cumsum([0] + [b.n for b in blocks[:-1]])
+        lists = []
+        for x in range(N_XCD):
+            parts = []
+            for j, (b0, blk) in enumerate(zip(base, blocks)):
+                li = blk.xcd_lists[(x - j) % N_XCD]
+                if len(li):
+                    parts.append(li + np.array([b0, 0], dtype=np.int32))
+            lists.append(np.concatenate(parts) if parts else np.zeros((0, 2), np.int32))
+        order = interleave_bins(lists)
+        self.n_items = int(len(order)) if sum(b.n_items for b in blocks) else 0
+        self.items = _pinned(order).to(dev, non_blocking=True)
 
-    def forward(self, stream=None, grid_waves: int = 0) -> None:
+    def forward(self, stream=None, grid_waves: int = DEFAULT_GRID_WAVES) -> None:
         lib, st = _lib.lib(), _stream(stream)
         _lib.check(lib.mc_attn_quantize_batched(_dptr(self.qk_jobs), 2 * self.n_mats, self.qk_bits, self.rounding,
                                                 self.max_count, st), "mc_attn_quantize_batched")
         _lib.check(lib.mc_attn_quantize_batched(_dptr(self.v_jobs), self.n_mats, 8, self.rounding, self.max_count,
                                                 st), "mc_attn_quantize_batched")
         _lib.check(lib.mc_attn_forward(_dptr(self.desc), _dptr(self.items), self.n_items, self.qk_bits, self.V,
-                                       self.p_scale, self.p_mode, grid_waves, st), "mc_attn_forward")
+                                       self.p_scale, self.p_mode, grid_waves, 0, st), "mc_attn_forward")
 
 
 # ---------------------------------------------------------------------------

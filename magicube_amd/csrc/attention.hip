@@ -176,7 +176,10 @@ __global__ __launch_bounds__(64 * WPB) void softmax_requant_kernel(int m_vec, co
 // logits, quantizes P, moves it to the A layout through LDS, gathers V rows
 // (4x4 byte transposes) and accumulates O = P V with MFMA.
 // ---------------------------------------------------------------------------
-template <int V, int QKB, int WPB>
+// DBG instantiations also write the optional int32 S, int8 P and int32 P V
+// outputs; the production ones carry no stores inside the chunk loops (a
+// conditional store there would make every load wait drain the queue).
+template <int V, int QKB, int WPB, bool DBG>
 __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *mats, const int2 *items, int n_items,
                                                              float p_scale, int p_mode) {
     constexpr int HOME = V < 4 ? V : 4;
@@ -191,6 +194,7 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
 
     for (int it = blockIdx.x * WPB + wave; it < n_items; it += nwaves) {
         const int2 item = items[it];
+        if (uniform(item.x) < 0) continue;           // padding slot of the work list
         const AttnMatrix Mv = mats[uniform(item.x)];
         AttnMatrix M;
         M.L = uniform(Mv.L);
@@ -224,26 +228,48 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
             if constexpr (QKB == 4) qop = expand_nibbles(qop, true, 0u);
         }
 
-        // logits of one 64-slot chunk in the MFMA C layout, scaled; masked -> -inf
-        auto chunk_logits = [&](int ch, float (&s)[4][4], int (&si)[4][4], int &my_idx) {
-            const int slot_l = start + 64 * ch + lane;                     // this lane's slot for index sharing
-            my_idx = slot_l < end ? M.col_indices[slot_l] : -1;
-            v4i acc[4];
+        // ---- software pipeline over 64-slot chunks.  Per half step for chunk
+        // ch: issue the index load of chunk ch+2, the row gathers of chunk ch+1
+        // (its index arrived one half step ago), then consume chunk ch.  Loads
+        // retire in order, so each consumer waits only for older loads and the
+        // gathers of the next chunk stay in flight.  Chunks past the row end
+        // are fully masked: they leave (m, l) bitwise unchanged and add P = 0,
+        // so the loops run in pairs without branches.
+        const __amdgpu_buffer_rsrc_t rc = make_rsrc(M.col_indices, 0x7FFFFFFFu);
+        auto load_idx = [&](int ch) {
+            const int slot = start + 64 * ch + lane;
+            const int v = (int)buf_load_b32(rc, slot < end ? (u32)slot * 4u : 0x80000000u);
+            return slot < end ? v : -1;
+        };
+        auto load_krows = [&](int my_idx, v4i (&kop)[4]) {
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int idx = __shfl(my_idx, 16 * t + c, 64);
                 const u32 off = idx >= 0 ? (u32)idx * QROW + (u32)(16 * g * QKB / 8) : 0x80000000u;
-                v4i kop = load_k16<QKB>(rk, off);
-                if constexpr (QKB == 4) kop = expand_nibbles(kop, true, 0u);
-                acc[t] = mfma_i8(qop, kop, v4i{0, 0, 0, 0});
+                kop[t] = load_k16<QKB>(rk, off);
             }
+        };
+        // V rows of slots 16g + p (p = 0..15), dword c (columns 4c..4c+3)
+        auto load_vrows = [&](int my_idx, u32 (&raw)[16]) {
+#pragma unroll
+            for (int p = 0; p < 16; ++p) {
+                const int idx = __shfl(my_idx, 16 * g + p, 64);
+                const u32 off = idx >= 0 ? (u32)idx * kHeadDim + 4u * c : 0x80000000u;
+                raw[p] = buf_load_b32(rv, off);
+            }
+        };
+        // logits of one chunk in the MFMA C layout, scaled; masked -> -inf
+        auto logits = [&](int ch, const v4i (&kop)[4], float (&s)[4][4], int (&si)[4][4]) {
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
+                v4i kk = kop[t];
+                if constexpr (QKB == 4) kk = expand_nibbles(kk, true, 0u);
+                const v4i acc = mfma_i8(qop, kk, v4i{0, 0, 0, 0});
                 const int slot = start + 64 * ch + 16 * t + c;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    si[t][i] = acc[t][i];
-                    s[t][i] = (home && i < HOME && slot < end) ? (float)acc[t][i] * M.alpha : -INFINITY;
+                    si[t][i] = acc[i];
+                    s[t][i] = (home && i < HOME && slot < end) ? (float)acc[i] * M.alpha : -INFINITY;
                 }
             }
         };
@@ -251,13 +277,12 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
         float m[HOME], l[HOME];
 #pragma unroll
         for (int i = 0; i < HOME; ++i) { m[i] = -INFINITY; l[i] = 0.f; }
-        for (int ch = 0; ch < nch; ++ch) {
+        auto pass1_chunk = [&](int ch, const v4i (&kop)[4]) {
             float s[4][4];
             int si[4][4];
-            int my_idx;
-            chunk_logits(ch, s, si, my_idx);
+            logits(ch, kop, s, si);
             softmax_chunk_update<HOME>(s, m, l);
-            if (M.s_out && home) {
+            if (DBG && M.s_out && home) {
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const int slot = start + 64 * ch + 16 * t + c;
@@ -265,6 +290,22 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
                     for (int i = 0; i < HOME; ++i)
                         if (slot < end) M.s_out[(size_t)slot * V + 4 * g + i] = si[t][i];
                 }
+            }
+        };
+        {
+            v4i ka[4], kb[4];
+            int ia = load_idx(0);
+            int ib = load_idx(1);
+            load_krows(ia, ka);
+            for (int ch = 0; ch < nch; ch += 2) {
+                ia = load_idx(ch + 2);
+                load_krows(ib, kb);
+                pass1_chunk(ch, ka);
+                __builtin_amdgcn_sched_barrier(0);
+                ib = load_idx(ch + 3);
+                load_krows(ia, ka);
+                pass1_chunk(ch + 1, kb);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         float inv_l[HOME];
@@ -274,11 +315,10 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
         v4i oacc[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) oacc[b] = v4i{0, 0, 0, 0};
-        for (int ch = 0; ch < nch_w; ++ch) {
+        auto pass2_chunk = [&](int ch, const v4i (&kop)[4], const u32 (&raw)[16]) {
             float s[4][4];
             int si[4][4];
-            int my_idx;
-            chunk_logits(ch, s, si, my_idx);
+            logits(ch, kop, s, si);
             int pq[4][4];
 #pragma unroll
             for (int t = 0; t < 4; ++t)
@@ -286,18 +326,10 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
                 for (int i = 0; i < 4; ++i)
                     pq[t][i] = i < HOME ? quantize_prob(s[t][i], m[i < HOME ? i : 0], inv_l[i < HOME ? i : 0], p_scale, p_mode) : 0;
             const v4i pop = p_to_a_layout<HOME>(tiles[wave], pq, lane, home);
-            if (M.p_out) {
+            if (DBG && M.p_out) {
                 const int slot0 = start + 64 * ch + 16 * g;
                 if (c < V && slot0 < aend)
                     *reinterpret_cast<v4i *>(M.p_out + (size_t)(slot0 >> 4) * 16 * V + 16 * c) = pop;
-            }
-            // B operand: V rows of slots 16g + p (p = 0..15), dword c (columns 4c..4c+3)
-            u32 raw[16];
-#pragma unroll
-            for (int p = 0; p < 16; ++p) {
-                const int idx = __shfl(my_idx, 16 * g + p, 64);
-                const u32 off = idx >= 0 ? (u32)idx * kHeadDim + 4u * c : 0x80000000u;
-                raw[p] = buf_load_b32(rv, off);
             }
             // after the 4x4 byte transposes, tr[q][b] holds column 4c + b of rows 4q..4q+3
             u32 tr[4][4];
@@ -308,13 +340,33 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
                 const v4i bop = v4i{(int)tr[0][b], (int)tr[1][b], (int)tr[2][b], (int)tr[3][b]};
                 oacc[b] = mfma_i8(pop, bop, oacc[b]);
             }
+        };
+        {
+            v4i ka[4], kb[4];
+            u32 va[16], vb[16];
+            int ia = load_idx(0);
+            int ib = load_idx(1);
+            load_krows(ia, ka);
+            load_vrows(ia, va);
+            for (int ch = 0; ch < nch_w; ch += 2) {
+                ia = load_idx(ch + 2);
+                load_krows(ib, kb);
+                load_vrows(ib, vb);
+                pass2_chunk(ch, ka, va);
+                __builtin_amdgcn_sched_barrier(0);
+                ib = load_idx(ch + 3);
+                load_krows(ia, ka);
+                load_vrows(ia, va);
+                pass2_chunk(ch + 1, kb, vb);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
         // rows 4g+i, columns 4c + b
         if (home) {
 #pragma unroll
             for (int i = 0; i < HOME; ++i) {
                 const size_t orow = (size_t)(i_row * V + 4 * g + i) * kHeadDim + 4 * c;
-                if (M.o_int)
+                if (DBG && M.o_int)
                     *reinterpret_cast<v4i *>(M.o_int + orow) = v4i{oacc[0][i], oacc[1][i], oacc[2][i], oacc[3][i]};
                 // same float operations as dequant_kernel with/without a row scale
                 const float rs = p_mode == kPModeReference ? 1.f : inv_l[i];
@@ -401,7 +453,7 @@ mc_status mc_attn_softmax_requant(int m_vec, int vec_length, const int *row_indi
 }
 
 mc_status mc_attn_forward(const mc_attn_matrix *matrices, const int *items, int n_items, int qk_bits, int vec_length,
-                          float p_scale, int p_mode, int grid_waves, hipStream_t stream) {
+                          float p_scale, int p_mode, int grid_waves, int debug, hipStream_t stream) {
     if (qk_bits != 8 && qk_bits != 4) return MC_EUNSUPPORTED;
     if (p_mode != MC_P_REFERENCE && p_mode != MC_P_ROW_SCALED) return MC_EINVAL_ARG;
     if (!mc::valid_v(vec_length)) return MC_EINVAL_V;
@@ -414,8 +466,15 @@ mc_status mc_attn_forward(const mc_attn_matrix *matrices, const int *items, int 
     const dim3 grid((unsigned)((waves + WPB - 1) / WPB)), block(64 * WPB);
     const auto *mats = reinterpret_cast<const mc::AttnMatrix *>(matrices);
     const auto *its = reinterpret_cast<const int2 *>(items);
-#define MC_ATTN_LAUNCH(V_, B_) \
-    hipLaunchKernelGGL((mc::attention_kernel<V_, B_, WPB>), grid, block, 0, stream, mats, its, n_items, p_scale, p_mode)
+#define MC_ATTN_LAUNCH(V_, B_)                                                                                       \
+    do {                                                                                                             \
+        if (debug)                                                                                                   \
+            hipLaunchKernelGGL((mc::attention_kernel<V_, B_, WPB, true>), grid, block, 0, stream, mats, its, n_items, \
+                               p_scale, p_mode);                                                                     \
+        else                                                                                                         \
+            hipLaunchKernelGGL((mc::attention_kernel<V_, B_, WPB, false>), grid, block, 0, stream, mats, its,        \
+                               n_items, p_scale, p_mode);                                                            \
+    } while (0)
     if (qk_bits == 8) {
         switch (vec_length) {
         case 2: MC_ATTN_LAUNCH(2, 8); break;

@@ -17,9 +17,9 @@
 //
 // Canonical softmax statistics: a row's slots are visited in chunks of 64
 // (four 16-slot MFMA tiles, slot order).  Per chunk: exact max M_c; running
-// max m' = max(m, M_c); chunk sum = (T0 + T1) + (T2 + T3) where each tile sum
-// T_t is a pairwise tree over the tile's 16 slots in slot order; running sum
-// l = l * exp(m - m') + chunk sum.  Both kernels evaluate these statistics
+// max m' = max(m, M_c); lane c first adds its four slots c, 16 + c, 32 + c,
+// 48 + c as (e0 + e1) + (e2 + e3), then the 16 lane sums are added as a
+// pairwise tree over lane order; running sum l = l * exp(m - m') + chunk sum.  Both kernels evaluate these statistics
 // on the same register layout (MFMA C layout: lane (c, g) holds rows 4g+i of
 // slot 16t + c) with the same instructions.
 #pragma once
@@ -30,16 +30,32 @@ namespace mc {
 
 constexpr int kHeadDim = 64;
 
-// Butterfly reductions within each 16-lane row of the wave.  Float addition
-// is commutative, so every lane of the row ends with the same bits.
+// Butterfly reductions within each 16-lane row of the wave, on DPP lane
+// moves (no LDS round trip): swap neighbours, swap pairs (quad_perm), then
+// combine the two quads of each 8-lane half (row_half_mirror) and the two
+// halves (row_mirror).  Each step adds the same two partial results in every
+// lane that holds them (in commuted order), so every lane of the row ends
+// with the same bits, equal to the xor butterfly over lane distances 1, 2,
+// 4, 8: ((x0+x1)+(x2+x3)) + ((x4+x5)+(x6+x7)) ... for a sum.
+template <int CTRL> __device__ __forceinline__ float dpp_move(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+constexpr int kDppSwap1 = 0xB1;        // quad_perm [1, 0, 3, 2]
+constexpr int kDppSwap2 = 0x4E;        // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141;  // lane i <- lane 7 - i within 8
+constexpr int kDppMirror = 0x140;      // lane i <- lane 15 - i within 16
 __device__ __forceinline__ float row16_max(float x) {
-#pragma unroll
-    for (int m = 1; m < 16; m <<= 1) x = fmaxf(x, __shfl_xor(x, m, 64));
+    x = fmaxf(x, dpp_move<kDppSwap1>(x));
+    x = fmaxf(x, dpp_move<kDppSwap2>(x));
+    x = fmaxf(x, dpp_move<kDppHalfMirror>(x));
+    x = fmaxf(x, dpp_move<kDppMirror>(x));
     return x;
 }
 __device__ __forceinline__ float row16_sum(float x) {
-#pragma unroll
-    for (int m = 1; m < 16; m <<= 1) x = x + __shfl_xor(x, m, 64);
+    x = x + dpp_move<kDppSwap1>(x);
+    x = x + dpp_move<kDppSwap2>(x);
+    x = x + dpp_move<kDppHalfMirror>(x);
+    x = x + dpp_move<kDppMirror>(x);
     return x;
 }
 
@@ -52,10 +68,10 @@ __device__ __forceinline__ void softmax_chunk_update(const float (&s)[4][4], flo
         float cm = fmaxf(fmaxf(s[0][i], s[1][i]), fmaxf(s[2][i], s[3][i]));
         cm = row16_max(cm);
         const float mn = fmaxf(m[i], cm);        // -inf only while every slot so far is masked
-        float tsum[4];
+        float e[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) tsum[t] = row16_sum(s[t][i] == -INFINITY ? 0.f : __expf(s[t][i] - mn));
-        const float csum = (tsum[0] + tsum[1]) + (tsum[2] + tsum[3]);
+        for (int t = 0; t < 4; ++t) e[t] = s[t][i] == -INFINITY ? 0.f : __expf(s[t][i] - mn);
+        const float csum = row16_sum((e[0] + e[1]) + (e[2] + e[3]));
         const float scale_old = (m[i] == -INFINITY) ? 0.f : __expf(m[i] - mn);
         l[i] = l[i] * scale_old + csum;
         m[i] = mn;

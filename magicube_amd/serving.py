@@ -103,7 +103,8 @@ class Server:
     """Admission, step assembly and retirement (see the module docstring)."""
 
     def __init__(self, hbm_budget: int, policy: str = "batched", V: int = 8, p_mode: int = attention.P_ROW_SCALED,
-                 p_scale: float = 127.0, max_inflight: int = 2, n_streams: int = 2, grid_waves: int = 0,
+                 p_scale: float = 127.0, max_inflight: int = 2, n_streams: int = 2,
+                 grid_waves: int = attention.DEFAULT_GRID_WAVES,
                  keep_outputs: bool = False, max_step_items: int | None = None, device="cuda"):
         if policy not in POLICIES:
             raise ValueError(f"policy must be one of {POLICIES}")

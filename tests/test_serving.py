@@ -56,10 +56,6 @@ def test_resident_block_descriptors_match_per_matrix_path():
         assert np.array_equal(blk.qk_jobs["count"], jobs["count"])
         assert np.array_equal(blk.qk_jobs["scale"], jobs["scale"])
         assert np.array_equal(blk.qk_jobs["x"], jobs["x"])
-        # work items: every (matrix, vector row) exactly once
-        items = blk.items_local.numpy()
-        exp = [(e, i) for e, d in enumerate(devs) for i in range(d.m_vec)]
-        assert sorted(map(tuple, items.tolist())) == exp
         outs = blk.outputs()
         assert [tuple(o.shape) for o in outs] == [(m.L, 64) for m in mats]
         assert [o.data_ptr() for o in outs] == [int(x) for x in blk.desc["o"]]
@@ -100,8 +96,9 @@ def test_block_batch_device_bytes_decode_to_valid_operands():
             assert 0 < cnt <= bb.max_count and inside(x, 2 * cnt) and inside(out, cnt * bits // 8)
         for x, cnt, _, out in np.frombuffer(raw(bb.v_jobs), dtype=attention._JOB_DT).tolist():
             assert inside(x, 2 * cnt) and inside(out, cnt)
-        items = bb.items.numpy().tolist()
-        assert len(set(map(tuple, items))) == bb.n_items == sum(int(d["m_vec"]) for d in desc)
+        items = [it for it in bb.items.numpy().tolist() if it[0] >= 0]
+        assert len(set(map(tuple, items))) == len(items) == sum(int(d["m_vec"]) for d in desc)
+        assert bb.n_items == bb.items.shape[0]
         for e, row in items:
             d = desc[e]
             L = int(d["L"])
@@ -113,3 +110,20 @@ def test_block_batch_device_bytes_decode_to_valid_operands():
             assert inside(int(d["o"]) + row * 8 * 256, 8 * 256)
             for f, size in (("q", L * 8 * bits), ("k", L * 8 * bits), ("v", L * 64)):
                 assert inside(int(d[f]), size)
+
+
+def test_xcd_item_order_covers_every_row_once_and_keeps_matrices_on_one_xcd():
+    import numpy as np
+    m_vecs = [5, 0, 37, 12, 64, 3, 9, 21, 18, 40, 1]
+    works = [10 * m + 7 * e for e, m in enumerate(m_vecs)]
+    order = attention.xcd_item_order(m_vecs, works, n_xcd=8, wpb=4)
+    real = order[order[:, 0] >= 0]
+    exp = sorted((e, r) for e, m in enumerate(m_vecs) for r in range(m))
+    assert sorted(map(tuple, real.tolist())) == exp
+    assert order.shape[0] % 32 == 0 and (order[order[:, 0] < 0] == -1).all()
+    xcd_of = {}
+    for pos, (e, _) in enumerate(order.tolist()):
+        if e >= 0:
+            x = (pos // 4) % 8
+            assert xcd_of.setdefault(e, x) == x, "a matrix spans two XCDs"
+    assert attention.xcd_item_order([0, 0], [0, 0]).shape == (1, 2)
