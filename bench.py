@@ -133,6 +133,57 @@ def cpu_baseline(csr, a_packed, rhs, M, K, N, V, budget_s=12.0):
     }
 
 
+def sddmm_bench(steps, warmup, seed=31):
+    """SDDMM at M = N = 4096, V = 8, K in {64, 128}, sparsity in {0.9, 0.95},
+    signed int8 and int4, reference output layout; exact check on sampled
+    vector rows; TOPS on unpadded nonzeros (2 * nnz_vec * V * K) and the
+    HBM roofline on compulsory bytes M*K*b/8 + N*K*b/8 + 4*nnz_vec +
+    4*(m_vec + 1) + 4*V*nnz_vec."""
+    import torch
+    from magicube_amd import formats, sddmm, synthetic
+    stream = torch.cuda.Stream()
+    M = N = 4096
+    V = 8
+    out = []
+    for bits in (8, 4):
+        for K in (64, 128):
+            for s in (0.9, 0.95):
+                csr = synthetic.uniform_block_mask(M, N, V, s, seed)
+                mask = sddmm.DeviceMask.upload(csr)
+                a = synthetic.random_lanes(M * K, bits, True, seed + 1)
+                b = synthetic.random_lanes(N * K, bits, True, seed + 2)
+                lhs = torch.from_numpy(formats.pack_lanes(a, bits).view(np.int32)).cuda()
+                rhs = torch.from_numpy(formats.pack_lanes(b, bits).view(np.int32)).cuda()
+                res = torch.zeros(mask.output_size(), dtype=torch.int32, device="cuda")
+
+                def run():
+                    mask.sddmm(bits, lhs, rhs, K, out=res, stream=stream)
+
+                with torch.cuda.stream(stream):
+                    run()
+                torch.cuda.synchronize()
+                got = res.cpu().numpy()
+                # exact check of 16 sampled vector rows
+                A = a.astype(np.int64).reshape(M, K)
+                B = b.astype(np.int64).reshape(N, K)
+                ok = True
+                for i in np.random.default_rng(seed).choice(csr.m_vec, 16, replace=False):
+                    s0, e0 = mask.pairs[2 * i], mask.pairs[2 * i + 1]
+                    cols = csr.col_indices[csr.row_offsets[i]:csr.row_offsets[i + 1]]
+                    exp = (A[i * V:(i + 1) * V] @ B[cols].T) & 0xFFFFFFFF
+                    j = np.arange(s0, e0)
+                    idx = (j[None, :] // 8) * 8 * V + 8 * np.arange(V)[:, None] + (j[None, :] % 8)
+                    ok &= bool(np.array_equal(got[idx].astype(np.int64) & 0xFFFFFFFF, exp))
+                ms, _ = time_gpu(run, steps, warmup, stream)
+                nnz = csr.nnz_vec
+                alg = M * K * bits // 8 + N * K * bits // 8 + 4 * nnz + 4 * (csr.m_vec + 1) + 4 * V * nnz
+                out.append({"bits": bits, "K": K, "sparsity": s, "nnz_vec": nnz, "us": ms * 1e3,
+                            "tops": 2.0 * nnz * V * K / (ms * 1e-3) / 1e12,
+                            "hbm_frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "verified": ok})
+    return {"workload": "SDDMM M=N=4096, V=8, uniform mask, signed int8/int4, reference output layout",
+            "cases": out}
+
+
 def dense_attention_fp64_torch(q, k, v, mask, V):
     """softmax(Q K^T / 8) V in float64 on the GPU over the vector-sparse mask
     (rows of a vector row with no column give zeros): the FP64 reference the
@@ -293,6 +344,7 @@ def main():
     ap.add_argument("--no-reference-shape", action="store_true")
     ap.add_argument("--no-attention", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-sddmm", action="store_true")
     ap.add_argument("--config", type=int, default=None, help="explicit SpMM tile config index")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 64 MiB device copy (counter calibration for rocprofv3 --pmc)")
@@ -369,6 +421,9 @@ def main():
                                      "ms_per_step": rms, "tops": rtops, "a100_published_tops": REFERENCE_A100_TOPS,
                                      "ratio_vs_a100": rtops / REFERENCE_A100_TOPS,
                                      "note": "synthetic pattern at the published DLMC shape"}
+
+    if not args.no_sddmm:
+        result["sddmm"] = sddmm_bench(max(1, args.steps // 2), max(1, args.warmup // 2))
 
     if not args.no_attention:
         result["attention"] = attention_bench(max(1, args.steps // 10), max(1, args.warmup // 4))
