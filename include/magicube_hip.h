@@ -87,6 +87,8 @@ mc_status mc_spmm_l4r4(int m_vec, int vec_length, int n, int k, const int *row_i
  *   multiple of 4*lane_dwords and (k+1)*4*lane_dwords bytes must fit in LDS),
  *   tiles_per_block waves per workgroup, split_k vector rows per workgroup,
  *   depth k-blocks of indices and A operands in flight per wave.
+ * waves_per_simd > 1: the kernel's registers are allocated so that this many
+ *   waves fit on a SIMD (0: the compiler's choice).
  * No reference counterpart (the reference fixes its tile per template,
  * SpMM/SpMM/src/wmma_spmm.cu:2137-2153); used by the autotuner. */
 typedef struct mc_tile_config {
@@ -95,6 +97,7 @@ typedef struct mc_tile_config {
     int split_k;
     int depth;
     int kind;
+    int waves_per_simd;
 } mc_tile_config;
 int mc_spmm_num_tile_configs(void);
 mc_status mc_spmm_get_tile_config(int index, mc_tile_config *out);

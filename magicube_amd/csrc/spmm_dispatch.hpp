@@ -9,16 +9,17 @@
 
 namespace mc {
 
-// Tile configurations {lane_dwords, tiles_per_block, split_k, depth, kind}:
-// 0-7 gather kernel; 8-11 B-stationary kernel (8-bit B), for which the fields
-// are {B dwords per slice row, waves per workgroup, vector rows per
-// workgroup, prefetch depth}.
-constexpr int kNumSpmmConfigs = 12;
-constexpr int kFirstStationaryConfig = 8;
+// Tile configurations {lane_dwords, tiles_per_block, split_k, depth, kind,
+// waves_per_simd}: 0-7 and 12 gather kernel; 8-11 B-stationary kernel (8-bit
+// B), for which the fields are {B dwords per slice row, waves per workgroup,
+// vector rows per workgroup, prefetch depth}.  waves_per_simd > 1 makes the
+// register allocation fit that many waves per SIMD.
+constexpr int kNumSpmmConfigs = 13;
 constexpr mc_tile_config kSpmmConfigs[kNumSpmmConfigs] = {
-    {2, 4, 1, 2, 0}, {2, 4, 2, 2, 0}, {4, 2, 2, 2, 0}, {4, 2, 4, 2, 0},
-    {1, 4, 2, 3, 0}, {2, 2, 2, 2, 0}, {4, 1, 4, 2, 0}, {1, 8, 1, 3, 0},
-    {8, 16, 32, 4, 1}, {8, 8, 32, 4, 1}, {16, 16, 32, 4, 1}, {8, 16, 16, 4, 1},
+    {2, 4, 1, 2, 0, 0}, {2, 4, 2, 2, 0, 0}, {4, 2, 2, 2, 0, 0}, {4, 2, 4, 2, 0, 0},
+    {1, 4, 2, 3, 0, 0}, {2, 2, 2, 2, 0, 0}, {4, 1, 4, 2, 0, 0}, {1, 8, 1, 3, 0, 0},
+    {8, 16, 32, 4, 1, 0}, {8, 8, 32, 4, 1, 0}, {16, 16, 32, 4, 1, 0}, {8, 16, 16, 4, 1, 0},
+    {1, 4, 1, 2, 0, 6},
 };
 
 // Whether a B-stationary config fits the shape (slice width divides n, the
@@ -29,7 +30,7 @@ inline bool stationary_fits(int cfg, int n, int k) {
     return c.kind == 1 && n % tb == 0 && (size_t)(k + 1) * tb <= (size_t)kStatLdsBytes;
 }
 
-template <int V, int AB, int BB, bool SGN, int LW, int TPB, int SK, int D>
+template <int V, int AB, int BB, bool SGN, int LW, int TPB, int SK, int D, int WPS = 1>
 mc_status spmm_launch_one(SpmmArgs a, hipStream_t stream) {
     constexpr int TILE_N = 16 * LW * (32 / BB);
     if ((a.n / (32 / BB)) % LW != 0) return MC_EUNSUPPORTED;
@@ -38,8 +39,8 @@ mc_status spmm_launch_one(SpmmArgs a, hipStream_t stream) {
     const long long blocks = (long long)a.m_vec * groups;
     if (blocks <= 0) return MC_OK;
     if (blocks > 0x7fffffffLL) return MC_EINVAL_SHAPE;
-    hipLaunchKernelGGL((spmm_kernel<V, AB, BB, SGN, LW, TPB, SK, D>), dim3((unsigned)blocks), dim3(64 * TPB * SK), 0,
-                       stream, a);
+    hipLaunchKernelGGL((spmm_kernel<V, AB, BB, SGN, LW, TPB, SK, D, WPS>), dim3((unsigned)blocks),
+                       dim3(64 * TPB * SK), 0, stream, a);
     return check_launch();
 }
 
@@ -72,6 +73,7 @@ mc_status spmm_launch_cfg(int cfg, const SpmmArgs &a, hipStream_t s) {
     case 11:
         if constexpr (BB == 8) return spmm_stationary_launch<V, AB, SGN, 32, 16, 4>(a, 16, s);
         else return MC_EUNSUPPORTED;
+    case 12: return spmm_launch_one<V, AB, BB, SGN, 1, 4, 1, 2, 6>(a, s);
     default: return MC_EINVAL_ARG;
     }
 }

@@ -152,8 +152,10 @@ constexpr int kStageSlots = 1024;
 // always lands in [2^30, 2^31), outside the buffer.
 constexpr u32 kOobAdd = 0x40000000u;
 
-template <int V, int AB, int BB, bool SGN, int LW, int TPB, int SPLITK, int DEPTH>
-__global__ __launch_bounds__(64 * TPB * SPLITK) void spmm_kernel(SpmmArgs args) {
+// WPS: waves per SIMD the register allocation must allow (1: no constraint).
+template <int V, int AB, int BB, bool SGN, int LW, int TPB, int SPLITK, int DEPTH, int WPS>
+__global__ __launch_bounds__(64 * TPB * SPLITK) __attribute__((amdgpu_waves_per_eu(WPS))) void spmm_kernel(
+    SpmmArgs args) {
     using T = Pair<AB, BB, SGN>;
     constexpr int P = T::P;
     constexpr int ROWS = P * V;

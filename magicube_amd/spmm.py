@@ -85,7 +85,7 @@ def tile_configs():
     lib = _lib.lib()
     out = []
     for i in range(lib.mc_spmm_num_tile_configs()):
-        buf = (ctypes.c_int * 5)()
+        buf = (ctypes.c_int * 6)()
         _lib.check(lib.mc_spmm_get_tile_config(i, buf), "mc_spmm_get_tile_config")
         out.append(tuple(buf))
     return out

@@ -88,7 +88,8 @@ def test_tile_config_table(mclib):
     from magicube_amd import spmm
     cfgs = spmm.tile_configs()
     assert len(cfgs) == mclib.mc_spmm_num_tile_configs() >= 4
-    assert all(c[4] in (0, 1) and c[1] >= 1 and c[2] >= 1 and c[3] >= 1 for c in cfgs)
+    assert all(c[4] in (0, 1) and c[1] >= 1 and c[2] >= 1 and c[3] >= 1 and 0 <= c[5] <= 10 for c in cfgs)
+    assert len(cfgs[0]) == 6
     assert all(c[0] in (1, 2, 4) for c in cfgs if c[4] == 0)
     assert all(c[0] in (8, 16) for c in cfgs if c[4] == 1)
     stationary = [i for i, c in enumerate(cfgs) if c[4] == 1]
