@@ -2,8 +2,9 @@
  * magicube_compat.hpp — C++ shim with the reference's entry-point names.
  *
  * Callers of Magicube's
- *   spmm::wmmaSpmm_{4b,8b4b,8b,16b8b}   (SpMM/SpMM/include/wmma_spmm.cuh:15-70)
- *   sddmm::wmmaSddmm_{4b,8b}            (SDDMM/SDDMM/include/wmma_sddmm.cuh:7-32)
+ *   spmm::wmmaSpmm_{4b,8b4b,12b4b,16b4b,8b,12b8b,16b8b,16b}
+ *                                       (SpMM/SpMM/include/wmma_spmm.cuh:15-78)
+ *   sddmm::wmmaSddmm_{4b,8b,16b}        (SDDMM/SDDMM/include/wmma_sddmm.cuh:7-32)
  * switch by including this header instead of the reference's and linking
  * libmagicube_hip.so.  Arguments and their order are unchanged.  The trailing
  * stream (default: null stream) and lane interpretation (default: unsigned,
@@ -49,6 +50,10 @@ MAGICUBE_SPMM_SHIM(wmmaSpmm_8b, mc_spmm_l8r8)
 MAGICUBE_SPMM_SHIM(wmmaSpmm_16b8b, mc_spmm_l16r8)
 MAGICUBE_SPMM_SHIM(wmmaSpmm_8b4b, mc_spmm_l8r4)
 MAGICUBE_SPMM_SHIM(wmmaSpmm_4b, mc_spmm_l4r4)
+MAGICUBE_SPMM_SHIM(wmmaSpmm_12b8b, mc_spmm_l12r8)
+MAGICUBE_SPMM_SHIM(wmmaSpmm_16b, mc_spmm_l16r16)
+MAGICUBE_SPMM_SHIM(wmmaSpmm_12b4b, mc_spmm_l12r4)
+MAGICUBE_SPMM_SHIM(wmmaSpmm_16b4b, mc_spmm_l16r4)
 #undef MAGICUBE_SPMM_SHIM
 
 }  // namespace spmm
@@ -65,6 +70,7 @@ namespace sddmm {
     }
 MAGICUBE_SDDMM_SHIM(wmmaSddmm_8b, mc_sddmm_l8r8)
 MAGICUBE_SDDMM_SHIM(wmmaSddmm_4b, mc_sddmm_l4r4)
+MAGICUBE_SDDMM_SHIM(wmmaSddmm_16b, mc_sddmm_l16r16)
 #undef MAGICUBE_SDDMM_SHIM
 
 }  // namespace sddmm

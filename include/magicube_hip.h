@@ -54,12 +54,14 @@ const char *mc_status_string(int status);
 int mc_last_hip_error(void);  /* hipError_t of the last MC_EHIP in this thread */
 
 /* ---- SpMM ----------------------------------------------------------------
- * Replace spmm::wmmaSpmm_8b / _16b8b / _8b4b / _4b
- * (SpMM/SpMM/include/wmma_spmm.cuh:15-70, launchers
- * SpMM/SpMM/src/wmma_spmm.cu:2091-2383): same arguments in the same order,
- * plus the lane interpretation and a stream.  n must be a multiple of 4
- * (8-bit B) or 8 (4-bit B); k*n*bits(B)/8 < 2^30.  row_indices may be NULL
- * (identity order). */
+ * Replace spmm::wmmaSpmm_8b / _16b8b / _12b8b / _16b / _8b4b / _12b4b /
+ * _16b4b / _4b (SpMM/SpMM/include/wmma_spmm.cuh:15-78, launchers
+ * SpMM/SpMM/src/wmma_spmm.cu:2091-2459): same arguments in the same order,
+ * plus the lane interpretation and a stream.  n must be a multiple of 2
+ * (16-bit B), 4 (8-bit B) or 8 (4-bit B); k*n*bits(B)/8 < 2^30.
+ * row_indices may be NULL (identity order).  12-bit A lanes sit in 16-bit
+ * containers (the reference's preA = 16, preA_cut = 12); bits 12..15 are
+ * ignored. */
 mc_status mc_spmm_l8r8(int m_vec, int vec_length, int n, int k, const int *row_indices,
                        const int *row_offsets, const int *column_indices, const int *values,
                        const int *rhs_matrix, int *output_matrix, mc_signedness sign,
@@ -76,6 +78,26 @@ mc_status mc_spmm_l4r4(int m_vec, int vec_length, int n, int k, const int *row_i
                        const int *row_offsets, const int *column_indices, const int *values,
                        const int *rhs_matrix, int *output_matrix, mc_signedness sign,
                        hipStream_t stream);
+/* spmm::wmmaSpmm_12b8b, SpMM/SpMM/include/wmma_spmm.cuh:55-61, wmma_spmm.cu:2385-2412 */
+mc_status mc_spmm_l12r8(int m_vec, int vec_length, int n, int k, const int *row_indices,
+                        const int *row_offsets, const int *column_indices, const int *values,
+                        const int *rhs_matrix, int *output_matrix, mc_signedness sign,
+                        hipStream_t stream);
+/* spmm::wmmaSpmm_16b (16-bit A and B), wmma_spmm.cuh:71-77, wmma_spmm.cu:2434-2459 */
+mc_status mc_spmm_l16r16(int m_vec, int vec_length, int n, int k, const int *row_indices,
+                         const int *row_offsets, const int *column_indices, const int *values,
+                         const int *rhs_matrix, int *output_matrix, mc_signedness sign,
+                         hipStream_t stream);
+/* spmm::wmmaSpmm_12b4b, wmma_spmm.cuh:31-37, wmma_spmm.cu:2208-2235 */
+mc_status mc_spmm_l12r4(int m_vec, int vec_length, int n, int k, const int *row_indices,
+                        const int *row_offsets, const int *column_indices, const int *values,
+                        const int *rhs_matrix, int *output_matrix, mc_signedness sign,
+                        hipStream_t stream);
+/* spmm::wmmaSpmm_16b4b, wmma_spmm.cuh:39-45, wmma_spmm.cu:2260-2287 */
+mc_status mc_spmm_l16r4(int m_vec, int vec_length, int n, int k, const int *row_indices,
+                        const int *row_offsets, const int *column_indices, const int *values,
+                        const int *rhs_matrix, int *output_matrix, mc_signedness sign,
+                        hipStream_t stream);
 
 /* Tile configurations of the SpMM kernels (an index into a fixed table).
  * kind 0, gather kernel: lane_dwords B dwords gathered per lane per row,
@@ -102,7 +124,8 @@ typedef struct mc_tile_config {
 int mc_spmm_num_tile_configs(void);
 mc_status mc_spmm_get_tile_config(int index, mc_tile_config *out);
 /* Generic SpMM with an explicit config index (-1: the table default for the
- * shape).  lhs_bits in {4, 8, 16}, rhs_bits in {4, 8}. */
+ * shape).  (lhs_bits, rhs_bits) in {(8,8), (12,8), (16,8), (16,16), (4,4),
+ * (8,4), (12,4), (16,4)}. */
 mc_status mc_spmm_run(int lhs_bits, int rhs_bits, mc_signedness sign, int config, int m_vec,
                       int vec_length, int n, int k, const int *row_indices,
                       const int *row_offsets, const int *column_indices, const int *values,
@@ -112,8 +135,8 @@ mc_status mc_spmm_run(int lhs_bits, int rhs_bits, mc_signedness sign, int config
 mc_status mc_spmm_set_tile_config(int lhs_bits, int rhs_bits, int vec_length, int n, int config);
 
 /* ---- SDDMM ----------------------------------------------------------------
- * Replace sddmm::wmmaSddmm_8b / _4b (SDDMM/SDDMM/include/wmma_sddmm.cuh:7-32,
- * launchers SDDMM/SDDMM/src/wmma_sddmm.cu:376-465): same arguments in the
+ * Replace sddmm::wmmaSddmm_8b / _4b / _16b (SDDMM/SDDMM/include/wmma_sddmm.cuh:7-32,
+ * launchers SDDMM/SDDMM/src/wmma_sddmm.cu:376-514): same arguments in the
  * same order (note k before n), plus the lane interpretation and a stream.
  *   out[j][v] = sum_k lhs[(i*V+v)][k] * rhs[col_j][k]
  * lhs is (m_vec*vec_length) x k, rhs n x k, both row-major packed; k is 64,
@@ -126,10 +149,14 @@ mc_status mc_sddmm_l8r8(int m_vec, int k, int n, const int *row_indices, const i
 mc_status mc_sddmm_l4r4(int m_vec, int k, int n, const int *row_indices, const int *row_offsets,
                         const int *col_indices, const int *lhs_matrix, const int *rhs_matrix,
                         int *output_values, int vec_length, mc_signedness sign, hipStream_t stream);
+/* sddmm::wmmaSddmm_16b (16-bit lhs and rhs), wmma_sddmm.cuh:25-32 */
+mc_status mc_sddmm_l16r16(int m_vec, int k, int n, const int *row_indices, const int *row_offsets,
+                          const int *col_indices, const int *lhs_matrix, const int *rhs_matrix,
+                          int *output_values, int vec_length, mc_signedness sign, hipStream_t stream);
 /* Output layouts of mc_sddmm_run. */
 #define MC_SDDMM_OUT_REFERENCE 0   /* [(j/8)][v][j%8] */
 #define MC_SDDMM_OUT_VECTOR 1      /* [j][v] (vector-major, the attention layout) */
-/* Generic SDDMM: bits in {4, 8} for both operands, explicit output layout. */
+/* Generic SDDMM: bits in {4, 8, 16} for both operands, explicit output layout. */
 mc_status mc_sddmm_run(int bits, mc_signedness sign, int layout, int m_vec, int k, int n,
                        const int *row_indices, const int *row_offsets, const int *col_indices,
                        const int *lhs_matrix, const int *rhs_matrix, int *output_values,
@@ -237,7 +264,9 @@ int mc_format_align_rows(int m_vec, const int *row_offsets, int alignment, int *
 mc_status mc_format_align_cols(int m_vec, const int *row_offsets, const int *col_indices,
                                const int *aligned_pairs, int aligned_num, int shuffle,
                                int *aligned_cols);
-/* Prepared A values: aligned_num*vec_length*lhs_bits/8 bytes into `out`. */
+/* Prepared A values: aligned_num*vec_length*c/8 bytes into `out`, where c is
+ * the lane container width (16 for 12-bit lanes, else lhs_bits); `values`
+ * holds nnz vectors of vec_length lanes at that width. */
 mc_status mc_format_prepare_values(int lhs_bits, int rhs_bits, int vec_length, int m_vec,
                                    const int *row_offsets, const int *aligned_pairs,
                                    int aligned_num, const void *values, void *out);

@@ -44,6 +44,7 @@ _SIGS = {
 }
 _SIGS["mc_sddmm_l8r8"] = (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _I, _P])
 _SIGS["mc_sddmm_l4r4"] = (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _I, _P])
+_SIGS["mc_sddmm_l16r16"] = (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _I, _P])
 _SIGS["mc_sddmm_run"] = (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P])
 _F = ctypes.c_float
 _LL = ctypes.c_longlong
@@ -52,7 +53,8 @@ _SIGS["mc_attn_absmax"] = (_I, [_P, _LL, _P, _P])
 _SIGS["mc_attn_softmax_requant"] = (_I, [_I, _I, _P, _P, _P, _F, _F, _I, _P, _P, _P])
 _SIGS["mc_attn_dequant"] = (_I, [_P, _I, _I, _F, _P, _P, _P])
 _SIGS["mc_attn_forward"] = (_I, [_P, _P, _I, _I, _I, _F, _I, _I, _I, _P])
-for _name in ("mc_spmm_l8r8", "mc_spmm_l16r8", "mc_spmm_l8r4", "mc_spmm_l4r4"):
+for _name in ("mc_spmm_l8r8", "mc_spmm_l16r8", "mc_spmm_l8r4", "mc_spmm_l4r4", "mc_spmm_l12r8", "mc_spmm_l16r16",
+              "mc_spmm_l12r4", "mc_spmm_l16r4"):
     _SIGS[_name] = (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P])
 
 

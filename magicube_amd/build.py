@@ -33,6 +33,10 @@ LIB_SOURCES = [
     "spmm_l16r8.hip",
     "spmm_l8r4.hip",
     "spmm_l4r4.hip",
+    "spmm_l12r8.hip",
+    "spmm_l16r16.hip",
+    "spmm_l12r4.hip",
+    "spmm_l16r4.hip",
 ]
 OPTIONAL_SOURCES = ["sddmm.hip", "attention.hip"]
 DRIVERS = {"spmm_benchmark": "spmm_benchmark.cpp", "sddmm_benchmark": "sddmm_benchmark.cpp"}

@@ -44,7 +44,7 @@ mc_status sddmm_launch_all(bool sgn, bool ref, int V, int nkc, const SddmmArgs &
 mc_status sddmm_entry(int bits, mc_signedness sign, int layout, int m_vec, int k, int n, const int *row_indices,
                       const int *row_offsets, const int *col_indices, const int *lhs_matrix, const int *rhs_matrix,
                       int *output_values, int vec_length, hipStream_t stream) {
-    if (bits != 8 && bits != 4) return MC_EUNSUPPORTED;
+    if (bits != 16 && bits != 8 && bits != 4) return MC_EUNSUPPORTED;
     if (sign != MC_SIGNED && sign != MC_UNSIGNED) return MC_EINVAL_ARG;
     if (layout != MC_SDDMM_OUT_REFERENCE && layout != MC_SDDMM_OUT_VECTOR) return MC_EINVAL_ARG;
     if (!valid_v(vec_length)) return MC_EINVAL_V;
@@ -69,6 +69,7 @@ mc_status sddmm_entry(int bits, mc_signedness sign, int layout, int m_vec, int k
     a.lhs_bytes = (unsigned)lhs_bytes;
     a.rhs_bytes = (unsigned)rhs_bytes;
     const bool sgn = sign == MC_SIGNED, ref = layout == MC_SDDMM_OUT_REFERENCE;
+    if (bits == 16) return sddmm_launch_all<16>(sgn, ref, vec_length, nkc, a, stream);
     return bits == 8 ? sddmm_launch_all<8>(sgn, ref, vec_length, nkc, a, stream)
                      : sddmm_launch_all<4>(sgn, ref, vec_length, nkc, a, stream);
 }
@@ -89,6 +90,13 @@ mc_status mc_sddmm_l4r4(int m_vec, int k, int n, const int *row_indices, const i
                         const int *col_indices, const int *lhs_matrix, const int *rhs_matrix, int *output_values,
                         int vec_length, mc_signedness sign, hipStream_t stream) {
     return mc::sddmm_entry(4, sign, MC_SDDMM_OUT_REFERENCE, m_vec, k, n, row_indices, row_offsets, col_indices,
+                           lhs_matrix, rhs_matrix, output_values, vec_length, stream);
+}
+
+mc_status mc_sddmm_l16r16(int m_vec, int k, int n, const int *row_indices, const int *row_offsets,
+                          const int *col_indices, const int *lhs_matrix, const int *rhs_matrix, int *output_values,
+                          int vec_length, mc_signedness sign, hipStream_t stream) {
+    return mc::sddmm_entry(16, sign, MC_SDDMM_OUT_REFERENCE, m_vec, k, n, row_indices, row_offsets, col_indices,
                            lhs_matrix, rhs_matrix, output_values, vec_length, stream);
 }
 

@@ -7,7 +7,7 @@
 //
 // bm is a DLMC .smtx file (its column count is the SDDMM's n) or
 // "synthetic:M,N,sparsity[,seed]"; k in {64, 128, 256}; (preA, preB) in
-// {(8,8), (4,4)}.  Rows are padded to 8 slots and the output uses the
+// {(16,16), (8,8), (4,4)}.  Rows are padded to 8 slots and the output uses the
 // reference layout [(j/8)][v][j%8]; `function` 1 verifies every output slot
 // (padding slots must be 0) against a host loop.
 #include <iostream>
@@ -23,7 +23,7 @@ int run(const std::string &bm, int K, int V, int sorted, int func, int sparse, i
         std::printf("Unsupported vec_length!\n");
         return 1;
     }
-    if (!((preA == 8 && preB == 8) || (preA == 4 && preB == 4)) || sparse != 1) {
+    if (!((preA == 16 && preB == 16) || (preA == 8 && preB == 8) || (preA == 4 && preB == 4)) || sparse != 1) {
         std::printf("Unsupported precision!\n");
         return 1;
     }
@@ -56,7 +56,7 @@ int run(const std::string &bm, int K, int V, int sorted, int func, int sparse, i
     mcb::DeviceBuffer<int> d_pairs(pairs.data(), 2 * (size_t)m_vec), d_cols(cols.data(), aligned),
         d_order(order.data(), m_vec), d_out((size_t)aligned * V);
     mcb::DeviceBuffer<uint8_t> d_lhs(lhs.data(), lhs_bytes), d_rhs(rhs.data(), rhs_bytes);
-    auto fn = bits == 8 ? mc_sddmm_l8r8 : mc_sddmm_l4r4;
+    auto fn = bits == 16 ? mc_sddmm_l16r16 : bits == 8 ? mc_sddmm_l8r8 : mc_sddmm_l4r4;
     auto launch = [&]() {
         mcb::check_mc(fn(m_vec, K, N, d_order.ptr, d_pairs.ptr, d_cols.ptr, reinterpret_cast<const int *>(d_lhs.ptr),
                          reinterpret_cast<const int *>(d_rhs.ptr), d_out.ptr, V, MC_UNSIGNED, nullptr),
@@ -104,7 +104,7 @@ int main(int argc, char **argv) {
         std::printf("sort    : 1 = longest rows first, 0 = in order\n");
         std::printf("function: 1 = verify every output, 0 = skip\n");
         std::printf("sparse  : 1 (SDDMM)\n");
-        std::printf("preA preB: 8 8 | 4 4\n");
+        std::printf("preA preB: 16 16 | 8 8 | 4 4\n");
         return argc == 2 ? 0 : 1;
     }
     const std::string bm(argv[1]);

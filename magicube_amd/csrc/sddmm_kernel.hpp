@@ -2,8 +2,8 @@
 //   out[j][v] = sum_k lhs[i*V + v][k] * rhs[col_j][k]   for every nonzero slot j
 // of vector row i (Magicube's wmmaSddmm, SDDMM/SDDMM/src/wmma_sddmm.cu:15-198).
 //
-// lhs is M x K and rhs is N x K, row-major, packed 4 x int8 or 8 x int4 per
-// int32 (lane 0 low).  Both operands are contiguous along K, so MFMA operands
+// lhs is M x K and rhs is N x K, row-major, packed 2 x int16, 4 x int8 or
+// 8 x int4 per int32 (lane 0 low).  Both operands are contiguous along K, so MFMA operands
 // are plain 16-byte (8-bit) or 8-byte (4-bit) loads: lane (r, g) takes the
 // 16 k-values [64*kc + 16g, +16) of lhs row r (A operand) and of the gathered
 // rhs row of nonzero r (B operand) — no transposes.  One MFMA covers 16
@@ -12,7 +12,13 @@
 // Digits as in the SpMM kernel: unsigned 8-bit lhs splits into two nibble
 // planes (MFMA rows plane*V + v); unsigned 8-bit rhs uses d = b ^ 0x80 with
 // offset +128, signed 4-bit rhs d = b ^ 8 with offset -8; the offset is
-// corrected with the lhs row sum over K.  Results are exact modulo 2^32.
+// corrected with the lhs row sum over K.  16-bit operands are two byte
+// digits on both sides (high x256, low x1; unsigned bytes and signed low
+// bytes as b ^ 0x80), so lhs = lhs' + OA and rhs = rhs' + OB, and
+//   sum lhs*rhs = sum lhs'*rhs' + OB*sum lhs' + OA*sum rhs' + K*OA*OB,
+// with sum rhs' taken per nonzero from the gathered operand.  The lhs digit
+// planes stack as MFMA rows (high plane rows 0..V-1); the two rhs digits are
+// two accumulators.  Results are exact modulo 2^32.
 //
 // Output slots: every slot of [start, roundup8(end)) is written (padding
 // slots get 0), in the reference layout [(j/8)][v][j%8] (OUT_REF) or
@@ -46,6 +52,24 @@ template <bool SGN> struct SddmmDigits<4, SGN> {
     static constexpr int P = 1, OB = SGN ? -8 : 0;
     __device__ static constexpr int weight(int) { return 1; }
 };
+template <bool SGN> struct SddmmDigits<16, SGN> {
+    static constexpr int P = 2, OB = SGN ? 128 : 32896, OA = OB;
+    __device__ static constexpr int weight(int p) { return p == 0 ? 256 : 1; }
+};
+
+// 16 k-values of 16-bit lanes at off_bytes as two byte-digit operands
+// (k order kept): hi = high bytes, lo = low bytes, each XORed with its mask.
+__device__ __forceinline__ void load_k16_w16(__amdgpu_buffer_rsrc_t r, u32 off_bytes, u32 xhi, u32 xlo, v4i &hi,
+                                             v4i &lo) {
+    const v4i w0 = buf_load_b128(r, off_bytes);
+    const v4i w1 = buf_load_b128(r, off_bytes + 16u);
+    const u32 w[8] = {(u32)w0[0], (u32)w0[1], (u32)w0[2], (u32)w0[3], (u32)w1[0], (u32)w1[1], (u32)w1[2], (u32)w1[3]};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        hi[q] = (int)(perm(w[2 * q + 1], w[2 * q], 0x07050301u) ^ xhi);
+        lo[q] = (int)(perm(w[2 * q + 1], w[2 * q], 0x06040200u) ^ xlo);
+    }
+}
 
 // 16 k-values of one packed row at k offset k0 (multiple of 16) as an int8
 // operand.  8-bit: 16 bytes.  4-bit: 8 bytes expanded to [even nibbles(x0),
@@ -76,6 +100,9 @@ __global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
     constexpr int P = D::P;
     constexpr int ROWS = P * V;
     constexpr int OB = D::OB;
+    constexpr bool W16 = BITS == 16;
+    // 16-bit digit masks: unsigned bytes and signed low bytes become b ^ 0x80
+    constexpr u32 XHI = SGN ? 0u : 0x80808080u, XLO = 0x80808080u;
     constexpr int HOME = V < 4 ? V : 4;
     constexpr int KB = 64 * BITS / 8;          // bytes of one 64-k chunk
     __shared__ __attribute__((aligned(16))) u32 s_off[kSddmmStage];
@@ -101,14 +128,23 @@ __global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
 #pragma unroll
     for (int kc = 0; kc < NKC; ++kc) {
         const u32 off = a_live ? (u32)(m * V + a_v) * row_bytes + (u32)(kc * KB + 16 * g * BITS / 8) : 0x80000000u;
-        v4i x = load_k16<BITS>(rl, off);
+        v4i x;
+        if constexpr (W16) {
+            v4i hi, lo;
+            load_k16_w16(rl, off, XHI, XLO, hi, lo);
+            // rows past 2V load nothing (offset out of range) but keep the
+            // XOR masks: zero them so every dead row sums to 0
+            x = a_live ? (a_p == 0 ? hi : lo) : v4i{0, 0, 0, 0};
+        } else {
+            x = load_k16<BITS>(rl, off);
+        }
         if constexpr (BITS == 8) {
             if constexpr (!SGN) {
                 const int sh = a_p == 0 ? 4 : 0;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) x[q] = (int)(((u32)x[q] >> sh) & 0x0F0F0F0Fu);
             }
-        } else {
+        } else if constexpr (BITS == 4) {
             x = expand_nibbles(x, SGN, 0u);
         }
         aop[kc] = x;
@@ -146,14 +182,25 @@ __global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
         (void)ntiles_total;
         // wave w handles tile groups w, w + W, ... of TPW tiles each
         for (int tg = wave * TPW; tg < ctiles; tg += W * TPW) {
-            v4i bop[TPW][NKC];
+            constexpr int BD = W16 ? 2 : 1;   // rhs digits (16-bit: low, high)
+            v4i bop[TPW][NKC][BD];
+            int bsum[TPW];                    // sum of rhs' over this lane's k (16-bit)
 #pragma unroll
             for (int t = 0; t < TPW; ++t) {
                 const int tile = min(tg + t, ctiles - 1);
                 const u32 add = (tg + t < ctiles) ? 0u : kSddmmOob;
                 const u32 o = s_off[tile * 16 + r] + add;
+                bsum[t] = 0;
 #pragma unroll
                 for (int kc = 0; kc < NKC; ++kc) {
+                    if constexpr (W16) {
+                        v4i hi, lo;
+                        load_k16_w16(rr, o + (u32)(kc * KB + 32 * g), XHI, XLO, hi, lo);
+                        bop[t][kc][0] = lo;
+                        bop[t][kc][BD - 1] = hi;
+                        bsum[t] += 256 * sum_bytes(hi) + sum_bytes(lo);
+                        continue;
+                    }
                     v4i x = load_k16<BITS>(rr, o + (u32)(kc * KB + 16 * g * BITS / 8));
                     if constexpr (BITS == 8) {
                         if constexpr (!SGN) {
@@ -163,30 +210,44 @@ __global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
                     } else {
                         x = expand_nibbles(x, false, SGN ? 0x88888888u : 0u);
                     }
-                    bop[t][kc] = x;
+                    bop[t][kc][0] = x;
                 }
             }
 #pragma unroll
             for (int t = 0; t < TPW; ++t) {
-                v4i acc = v4i{0, 0, 0, 0};
+                v4i acc[BD];
 #pragma unroll
-                for (int kc = 0; kc < NKC; ++kc) acc = mfma_i8(aop[kc], bop[t][kc], acc);
-                // recombine digit planes into the home lanes (rows v = 4g + i)
+                for (int q = 0; q < BD; ++q) {
+                    acc[q] = v4i{0, 0, 0, 0};
+#pragma unroll
+                    for (int kc = 0; kc < NKC; ++kc) acc[q] = mfma_i8(aop[kc], bop[t][kc][q], acc[q]);
+                }
+                // rhs digit sum of nonzero r over all of K (16-bit only)
+                u32 corr = 0;
+                if constexpr (W16) {
+                    int sb = bsum[t];
+                    sb += lane_read(sb, lane ^ 16);
+                    sb += lane_read(sb, lane ^ 32);
+                    corr = (u32)D::OA * (u32)sb + (u32)args.k * (u32)D::OA * (u32)OB;
+                }
+                // recombine digit planes into the home lanes (rows v = 4g + i),
+                // mod 2^32 in unsigned arithmetic
                 int res[HOME];
 #pragma unroll
                 for (int i = 0; i < HOME; ++i) {
-                    int vsum = OB * rs_home[i];
+                    u32 vsum = (u32)OB * (u32)rs_home[i] + corr;
 #pragma unroll
                     for (int p = 0; p < P; ++p) {
                         int dg, di;
                         if constexpr (V == 8) { dg = 2 * (p % 2); di = 0; }
                         else if constexpr (V == 4) { dg = p % 4; di = 0; }
                         else { dg = (p % 8) / 2; di = 2 * (p % 2); }
-                        int x = acc[i + di];
-                        if (dg != 0) x = lane_down(x, 16 * dg);
-                        vsum += D::weight(p) * x;
+                        u32 x = (u32)acc[0][i + di];
+                        if constexpr (W16) x += (u32)acc[1][i + di] << 8;
+                        if (dg != 0) x = (u32)lane_down((int)x, 16 * dg);
+                        vsum += (u32)D::weight(p) * x;
                     }
-                    res[i] = vsum;
+                    res[i] = (int)vsum;
                 }
                 const int j = c0 + (tg + t) * 16 + r;
                 if (g < (V + 3) / 4 && tg + t < ctiles && j < wend) {

@@ -22,19 +22,25 @@ int default_config(int lhs_bits, int rhs_bits, int V, int n) {
     // measured on MI355X at M=K=4096, N in {128..512}, sparsity 0.7-0.98: one
     // dword per lane with three k-blocks in flight (8-bit B) / two waves per
     // tile (4-bit B); the autotuner picks per shape among the others
-    return rhs_bits == 8 ? 7 : 4;
+    return rhs_bits == 4 ? 4 : 7;
 }
 
-// Four dwords per lane are built for 8-bit B only (4-bit B would hold 32
-// accumulator tiles per wave); the B-stationary kernel is built for 8-bit B.
+// Four dwords per lane are built for 8- and 16-bit B only (4-bit B would
+// hold 32 accumulator tiles per wave); the B-stationary kernel is built for
+// 8-bit B.
 bool config_supported(int cfg, int rhs_bits) {
     if (kSpmmConfigs[cfg].kind == 1) return rhs_bits == 8;
-    return rhs_bits == 8 || kSpmmConfigs[cfg].lane_dwords < 4;
+    return rhs_bits != 4 || kSpmmConfigs[cfg].lane_dwords < 4;
 }
 
+// The reference's precision table (SpMM/SpMM/spmm_benchmark.cpp:805-831).
 bool pair_supported(int lhs_bits, int rhs_bits) {
-    return (lhs_bits == 8 && rhs_bits == 8) || (lhs_bits == 16 && rhs_bits == 8) ||
-           (lhs_bits == 8 && rhs_bits == 4) || (lhs_bits == 4 && rhs_bits == 4);
+    switch (rhs_bits) {
+    case 4: return lhs_bits == 4 || lhs_bits == 8 || lhs_bits == 12 || lhs_bits == 16;
+    case 8: return lhs_bits == 8 || lhs_bits == 12 || lhs_bits == 16;
+    case 16: return lhs_bits == 16;
+    default: return false;
+    }
 }
 
 mc_status spmm_entry(int lhs_bits, int rhs_bits, mc_signedness sign, int config, int m_vec, int vec_length, int n,
@@ -77,7 +83,11 @@ mc_status spmm_entry(int lhs_bits, int rhs_bits, mc_signedness sign, int config,
     const bool sgn = sign == MC_SIGNED;
     if (lhs_bits == 8 && rhs_bits == 8) return spmm_launch_l8r8(sgn, vec_length, cfg, a, stream);
     if (lhs_bits == 16 && rhs_bits == 8) return spmm_launch_l16r8(sgn, vec_length, cfg, a, stream);
+    if (lhs_bits == 12 && rhs_bits == 8) return spmm_launch_l12r8(sgn, vec_length, cfg, a, stream);
+    if (lhs_bits == 16 && rhs_bits == 16) return spmm_launch_l16r16(sgn, vec_length, cfg, a, stream);
     if (lhs_bits == 8 && rhs_bits == 4) return spmm_launch_l8r4(sgn, vec_length, cfg, a, stream);
+    if (lhs_bits == 12 && rhs_bits == 4) return spmm_launch_l12r4(sgn, vec_length, cfg, a, stream);
+    if (lhs_bits == 16 && rhs_bits == 4) return spmm_launch_l16r4(sgn, vec_length, cfg, a, stream);
     return spmm_launch_l4r4(sgn, vec_length, cfg, a, stream);
 }
 
@@ -111,6 +121,34 @@ mc_status mc_spmm_l4r4(int m_vec, int vec_length, int n, int k, const int *row_i
                        const int *column_indices, const int *values, const int *rhs_matrix, int *output_matrix,
                        mc_signedness sign, hipStream_t stream) {
     return mc::spmm_entry(4, 4, sign, -1, m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values,
+                          rhs_matrix, output_matrix, stream);
+}
+
+mc_status mc_spmm_l12r8(int m_vec, int vec_length, int n, int k, const int *row_indices, const int *row_offsets,
+                       const int *column_indices, const int *values, const int *rhs_matrix, int *output_matrix,
+                       mc_signedness sign, hipStream_t stream) {
+    return mc::spmm_entry(12, 8, sign, -1, m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values,
+                          rhs_matrix, output_matrix, stream);
+}
+
+mc_status mc_spmm_l16r16(int m_vec, int vec_length, int n, int k, const int *row_indices, const int *row_offsets,
+                       const int *column_indices, const int *values, const int *rhs_matrix, int *output_matrix,
+                       mc_signedness sign, hipStream_t stream) {
+    return mc::spmm_entry(16, 16, sign, -1, m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values,
+                          rhs_matrix, output_matrix, stream);
+}
+
+mc_status mc_spmm_l12r4(int m_vec, int vec_length, int n, int k, const int *row_indices, const int *row_offsets,
+                       const int *column_indices, const int *values, const int *rhs_matrix, int *output_matrix,
+                       mc_signedness sign, hipStream_t stream) {
+    return mc::spmm_entry(12, 4, sign, -1, m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values,
+                          rhs_matrix, output_matrix, stream);
+}
+
+mc_status mc_spmm_l16r4(int m_vec, int vec_length, int n, int k, const int *row_indices, const int *row_offsets,
+                       const int *column_indices, const int *values, const int *rhs_matrix, int *output_matrix,
+                       mc_signedness sign, hipStream_t stream) {
+    return mc::spmm_entry(16, 4, sign, -1, m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values,
                           rhs_matrix, output_matrix, stream);
 }
 

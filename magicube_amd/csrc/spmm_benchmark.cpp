@@ -7,8 +7,10 @@
 //
 // bm is a DLMC .smtx file or "synthetic:M,K,sparsity[,seed]".  kernel 0 and
 // sparse 1 select the SpMM (the reference's other kernels are baselines that
-// this build does not carry); pairs (preA, preB) in {(8,8), (16,8), (8,4),
-// (4,4)}.  Operand values follow the reference's generator, B is K x n, the
+// this build does not carry); pairs (preA, preB) in {(8,8), (12,8), (16,8),
+// (16,16), (4,4), (8,4), (12,4), (16,4)}, 12-bit lanes in 16-bit containers
+// as the reference holds them.  Operand values follow the reference's
+// generator (MakeDenseMatrix of the reference's TypeA for the pair), B is K x n, the
 // lanes are unsigned as in the reference's MMAs, and `function` 1 verifies
 // every output element against a host loop over the unpadded pattern.
 #include <cinttypes>
@@ -32,6 +34,10 @@ bool pair_for(int preA, int preB, PairInfo &p) {
     else if (preA == 16 && preB == 8) p = {16, 8, mc_spmm_l16r8};
     else if (preA == 8 && preB == 4) p = {8, 4, mc_spmm_l8r4};
     else if (preA == 4 && preB == 4) p = {4, 4, mc_spmm_l4r4};
+    else if (preA == 12 && preB == 8) p = {12, 8, mc_spmm_l12r8};
+    else if (preA == 16 && preB == 16) p = {16, 16, mc_spmm_l16r16};
+    else if (preA == 12 && preB == 4) p = {12, 4, mc_spmm_l12r4};
+    else if (preA == 16 && preB == 4) p = {16, 4, mc_spmm_l16r4};
     else return false;
     return true;
 }
@@ -46,7 +52,9 @@ std::vector<uint32_t> host_spmm(const Pattern &pat, int V, int N, int abits, int
             const size_t col = (size_t)pat.col_indices[j];
             for (int n = 0; n < N; ++n) brow[n] = mcb::lane_at(b, col * N + n, bbits);
             for (int v = 0; v < V; ++v) {
-                const uint32_t av = mcb::lane_at(a, (size_t)j * V + v, abits);
+                // 12-bit lanes: low 12 bits of a 16-bit container
+                const uint32_t av = abits == 12 ? mcb::lane_at(a, (size_t)j * V + v, 16) & 0xFFFu
+                                                : mcb::lane_at(a, (size_t)j * V + v, abits);
                 uint32_t *crow = c.data() + ((size_t)i * V + v) * N;
                 for (int n = 0; n < N; ++n) crow[n] += av * brow[n];
             }
@@ -67,7 +75,7 @@ int run(const std::string &bm, int N, int V, int kernel, int sorted, int func, i
     const Pattern pat = mcb::load_pattern(bm, V);
     const int m_vec = pat.m_vec, K = pat.k, nnz = pat.nnz();
     const int mma_k = mc_format_row_alignment(pi.lhs_bits, pi.rhs_bits);
-    std::printf("preA %d, preA_cut %d, preB %d, vec_length %d \n", preA, preA, preB, V);
+    std::printf("preA %d, preA_cut %d, preB %d, vec_length %d \n", preA == 12 ? 16 : preA, preA, preB, V);
     std::printf("m_vec %d, dimN %d, nonzeros_vec %d, dimk %d, mma_k_dim %d \n", m_vec, N, nnz, K, mma_k);
 
     std::vector<int> pairs(2 * (size_t)std::max(m_vec, 1));
@@ -82,9 +90,11 @@ int run(const std::string &bm, int N, int V, int kernel, int sorted, int func, i
                                        pi.rhs_bits == 4, cols.data()),
                   "mc_format_align_cols");
 
-    // values: nnz vectors of V lanes; B: K x N lanes (reference generator)
-    const int vec_bytes = V * pi.lhs_bits / 8;
-    const int word = std::min(8, vec_bytes);
+    // values: nnz vectors of V lanes; B: K x N lanes (reference generator,
+    // TypeA short for L16-R16, else the widest word up to 8 bytes:
+    // SpMM/SpMM/spmm_benchmark.cpp:805-831)
+    const int vec_bytes = V * (pi.lhs_bits == 12 ? 16 : pi.lhs_bits) / 8;
+    const int word = pi.rhs_bits == 16 ? 2 : std::min(8, vec_bytes);
     std::vector<uint8_t> values((size_t)nnz * vec_bytes + 8);
     std::default_random_engine gen;
     mcb::fill_words(gen, word, (size_t)nnz * vec_bytes / word, values.data());
@@ -141,7 +151,7 @@ int main(int argc, char **argv) {
         std::printf("sort    : 1 = longest rows first, 0 = in order\n");
         std::printf("function: 1 = verify every output, 0 = skip\n");
         std::printf("sparse  : 1 (SpMM)\n");
-        std::printf("preA preB: 8 8 | 16 8 | 8 4 | 4 4\n");
+        std::printf("preA preB: 8 8 | 12 8 | 16 8 | 16 16 | 4 4 | 8 4 | 12 4 | 16 4\n");
         return argc == 2 ? 0 : 1;
     }
     const std::string bm(argv[1]);

@@ -10,8 +10,8 @@
 namespace mc {
 
 // Tile configurations {lane_dwords, tiles_per_block, split_k, depth, kind,
-// waves_per_simd}: 0-7 and 12 gather kernel; 8-11 B-stationary kernel (8-bit
-// B), for which the fields are {B dwords per slice row, waves per workgroup,
+// waves_per_simd}: 0-7 and 12 gather kernel (four dwords per lane: 8- and
+// 16-bit B); 8-11 B-stationary kernel (8-bit B), for which the fields are {B dwords per slice row, waves per workgroup,
 // vector rows per workgroup, prefetch depth}.  waves_per_simd > 1 makes the
 // register allocation fit that many waves per SIMD.
 constexpr int kNumSpmmConfigs = 13;
@@ -50,15 +50,15 @@ mc_status spmm_launch_cfg(int cfg, const SpmmArgs &a, hipStream_t s) {
     case 0: return spmm_launch_one<V, AB, BB, SGN, 2, 4, 1, 2>(a, s);
     case 1: return spmm_launch_one<V, AB, BB, SGN, 2, 4, 2, 2>(a, s);
     case 2:
-        if constexpr (BB == 8) return spmm_launch_one<V, AB, BB, SGN, 4, 2, 2, 2>(a, s);
+        if constexpr (BB != 4) return spmm_launch_one<V, AB, BB, SGN, 4, 2, 2, 2>(a, s);
         else return MC_EUNSUPPORTED;
     case 3:
-        if constexpr (BB == 8) return spmm_launch_one<V, AB, BB, SGN, 4, 2, 4, 2>(a, s);
+        if constexpr (BB != 4) return spmm_launch_one<V, AB, BB, SGN, 4, 2, 4, 2>(a, s);
         else return MC_EUNSUPPORTED;
     case 4: return spmm_launch_one<V, AB, BB, SGN, 1, 4, 2, 3>(a, s);
     case 5: return spmm_launch_one<V, AB, BB, SGN, 2, 2, 2, 2>(a, s);
     case 6:
-        if constexpr (BB == 8) return spmm_launch_one<V, AB, BB, SGN, 4, 1, 4, 2>(a, s);
+        if constexpr (BB != 4) return spmm_launch_one<V, AB, BB, SGN, 4, 1, 4, 2>(a, s);
         else return MC_EUNSUPPORTED;
     case 7: return spmm_launch_one<V, AB, BB, SGN, 1, 8, 1, 3>(a, s);
     case 8:
@@ -100,5 +100,9 @@ mc_status spmm_launch_l8r8(bool sgn, int V, int cfg, const SpmmArgs &a, hipStrea
 mc_status spmm_launch_l16r8(bool sgn, int V, int cfg, const SpmmArgs &a, hipStream_t s);
 mc_status spmm_launch_l8r4(bool sgn, int V, int cfg, const SpmmArgs &a, hipStream_t s);
 mc_status spmm_launch_l4r4(bool sgn, int V, int cfg, const SpmmArgs &a, hipStream_t s);
+mc_status spmm_launch_l12r8(bool sgn, int V, int cfg, const SpmmArgs &a, hipStream_t s);
+mc_status spmm_launch_l12r4(bool sgn, int V, int cfg, const SpmmArgs &a, hipStream_t s);
+mc_status spmm_launch_l16r4(bool sgn, int V, int cfg, const SpmmArgs &a, hipStream_t s);
+mc_status spmm_launch_l16r16(bool sgn, int V, int cfg, const SpmmArgs &a, hipStream_t s);
 
 }  // namespace mc

@@ -19,13 +19,19 @@
 // (row = plane*V + v, padded to 16-row tiles), and the planes are recombined
 // by shifted int32 adds in the epilogue.  B digits may carry a constant offset
 // OB (u8 -> b^0x80 = b-128, signed i4 -> nibble^8 = b+8), corrected with
-// OB * (row sum of A), computed on the A operand with v_dot4.
+// OB * (row sum of A), computed on the A operand with v_dot4.  16-bit B
+// columns are two byte digits (high x256, low x1), each its own MFMA column
+// group, recombined in the epilogue.  12-bit A sits in 16-bit containers.
 //
 //   pair    signed                          unsigned
 //   L8-R8   A s8                             A = 16*hi4 + lo4, B = d + 128
 //   L16-R8  A = 256*s8 + 16*u4 + u4          A = 4 nibbles,  B = d + 128
+//   L12-R8  A = 16*s8 + u4                   A = 3 nibbles,  B = d + 128
+//   L16-R16 A as L16-R8, B = 256*s8 + d + 128  A = 4 nibbles, B = 256*d' + d + 32896
 //   L8-R4   A s8 (from the two nibble        A = 16*hi4 + lo4
 //           planes), B = d - 8
+//   L16-R4  A = 256*s8 + 16*u4 + u4, B = d - 8   A = 4 nibbles, B u4
+//   L12-R4  A = 16*s8 + u4, B = d - 8        A = 3 nibbles,  B u4
 //   L4-R4   A s4->s8, B = d - 8              A u4, B u4
 //
 // The output is the exact sum modulo 2^32.
@@ -70,12 +76,37 @@ template <bool SGN> struct Pair<4, 4, SGN> {
     __device__ static constexpr int weight(int) { return 1; }
 };
 
+// 12-bit A: unsigned as three nibbles; signed as the arithmetic top byte
+// (bits 4..11, x16) and the low nibble.
+template <bool SGN> struct Pair<12, 8, SGN> {
+    static constexpr int MMA_K = 16, P = SGN ? 2 : 3, OB = SGN ? 0 : 128;
+    __device__ static constexpr int weight(int p) { return SGN ? (p == 0 ? 16 : 1) : (p == 0 ? 256 : p == 1 ? 16 : 1); }
+};
+template <bool SGN> struct Pair<12, 4, SGN> {
+    static constexpr int MMA_K = 32, P = SGN ? 2 : 3, OB = SGN ? -8 : 0;
+    __device__ static constexpr int weight(int p) { return SGN ? (p == 0 ? 16 : 1) : (p == 0 ? 256 : p == 1 ? 16 : 1); }
+};
+template <bool SGN> struct Pair<16, 4, SGN> {
+    static constexpr int MMA_K = 32, P = SGN ? 3 : 4, OB = SGN ? -8 : 0;
+    __device__ static constexpr int weight(int p) {
+        return SGN ? (p == 0 ? 256 : p == 1 ? 16 : 1) : (p == 0 ? 4096 : p == 1 ? 256 : p == 2 ? 16 : 1);
+    }
+};
+// 16-bit B: the offset covers both byte digits (unsigned: 128*256 + 128;
+// signed: the low byte only).
+template <bool SGN> struct Pair<16, 16, SGN> {
+    static constexpr int MMA_K = 16, P = SGN ? 3 : 4, OB = SGN ? 128 : 32896;
+    __device__ static constexpr int weight(int p) {
+        return SGN ? (p == 0 ? 256 : p == 1 ? 16 : 1) : (p == 0 ? 4096 : p == 1 ? 256 : p == 2 ? 16 : 1);
+    }
+};
+
 // A operand of one lane: plane p, vector lane v, reduction slots
 // [slot0, slot0 + 16).  Returns zeros when !ok (buffer offset out of range).
 template <int V, int AB, int BB, bool SGN>
 __device__ __forceinline__ v4i load_a_operand(__amdgpu_buffer_rsrc_t va, int slot0, int p, int v, bool ok) {
     const u32 OOB = 0x80000000u;
-    if constexpr (BB == 8) {
+    if constexpr (BB != 4) {
         // groups of 16 slots; [V][16] bytes per byte plane
         const int grp = slot0 >> 4;
         if constexpr (AB == 8) {
@@ -90,7 +121,7 @@ __device__ __forceinline__ v4i load_a_operand(__amdgpu_buffer_rsrc_t va, int slo
                 for (int q = 0; q < 4; ++q) y[q] = (int)(((u32)x[q] >> sh) & 0x0F0F0F0Fu);
                 return y;
             }
-        } else {  // AB == 16: low-byte plane then high-byte plane per group
+        } else if constexpr (AB == 16) {  // low-byte plane then high-byte plane per group
             const bool hi = SGN ? (p == 0) : (p < 2);
             const u32 off = ok ? (u32)(grp * 32 * V + (hi ? 16 * V : 0) + 16 * v) : OOB;
             v4i x = buf_load_b128(va, off);
@@ -100,6 +131,28 @@ __device__ __forceinline__ v4i load_a_operand(__amdgpu_buffer_rsrc_t va, int slo
 #pragma unroll
             for (int q = 0; q < 4; ++q) y[q] = (int)(((u32)x[q] >> sh) & 0x0F0F0F0Fu);
             return y;
+        } else {  // AB == 12: the 16-bit layout; the high-byte plane holds bits 8..11
+            const u32 base = (u32)(grp * 32 * V + 16 * v);
+            if constexpr (SGN) {
+                // digit 0 = bits 4..11 (signed byte), digit 1 = bits 0..3
+                v4i lo = buf_load_b128(va, ok ? base : OOB);
+                v4i hi = buf_load_b128(va, (ok && p == 0) ? base + 16 * V : OOB);
+                v4i y;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const u32 top = (((u32)hi[q] << 4) & 0xF0F0F0F0u) | (((u32)lo[q] >> 4) & 0x0F0F0F0Fu);
+                    y[q] = (int)(p == 0 ? top : ((u32)lo[q] & 0x0F0F0F0Fu));
+                }
+                return y;
+            } else {
+                // digits: bits 8..11 (high plane), 4..7, 0..3 (low plane)
+                v4i x = buf_load_b128(va, ok ? base + (p == 0 ? 16 * V : 0) : OOB);
+                const int sh = (p == 1) ? 4 : 0;
+                v4i y;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) y[q] = (int)(((u32)x[q] >> sh) & 0x0F0F0F0Fu);
+                return y;
+            }
         }
     } else {
         // groups of 32 slots; the lane's 16 slots are half h of the group.
@@ -113,7 +166,7 @@ __device__ __forceinline__ v4i load_a_operand(__amdgpu_buffer_rsrc_t va, int slo
             u32 e1 = nib_even((u32)x[1]), o1 = nib_odd((u32)x[1]);
             if constexpr (SGN) { e0 = sext4_bytes(e0); o0 = sext4_bytes(o0); e1 = sext4_bytes(e1); o1 = sext4_bytes(o1); }
             return v4i{(int)e0, (int)o0, (int)e1, (int)o1};
-        } else {  // AB == 8 with 4-bit B: low-nibble plane then high-nibble plane
+        } else if constexpr (AB == 8) {  // low-nibble plane then high-nibble plane
             const u32 base = (u32)(grp * 32 * V + 16 * v + 8 * h);
             if constexpr (SGN) {
                 v2i lo = buf_load_b64(va, ok ? base : OOB);
@@ -125,6 +178,27 @@ __device__ __forceinline__ v4i load_a_operand(__amdgpu_buffer_rsrc_t va, int slo
                 return v4i{(int)e0, (int)o0, (int)e1, (int)o1};
             } else {
                 v2i x = buf_load_b64(va, ok ? base + (p == 0 ? 16 * V : 0) : OOB);
+                return v4i{(int)nib_even((u32)x[0]), (int)nib_odd((u32)x[0]),
+                           (int)nib_even((u32)x[1]), (int)nib_odd((u32)x[1])};
+            }
+        } else {
+            // AB == 12 or 16: nibble planes q = 0..AB/4-1 (bits 4q..4q+3) at
+            // 16*V*q in a 64*V-byte group (the 12-bit layout leaves plane 3 empty)
+            constexpr int TOP = AB / 4 - 1;
+            const u32 base = (u32)(grp * 64 * V + 16 * v + 8 * h);
+            if constexpr (SGN) {
+                // digit 0 = planes TOP (high nibble) and TOP-1 as one signed
+                // byte; digit p >= 1 = plane TOP-1-p
+                const int plo = p == 0 ? TOP - 1 : TOP - 1 - p;
+                v2i lo = buf_load_b64(va, ok ? base + (u32)(16 * V * plo) : OOB);
+                v2i hi = buf_load_b64(va, (ok && p == 0) ? base + (u32)(16 * V * TOP) : OOB);
+                u32 e0 = nib_even((u32)lo[0]) | (nib_even((u32)hi[0]) << 4);
+                u32 o0 = nib_odd((u32)lo[0]) | (nib_odd((u32)hi[0]) << 4);
+                u32 e1 = nib_even((u32)lo[1]) | (nib_even((u32)hi[1]) << 4);
+                u32 o1 = nib_odd((u32)lo[1]) | (nib_odd((u32)hi[1]) << 4);
+                return v4i{(int)e0, (int)o0, (int)e1, (int)o1};
+            } else {
+                v2i x = buf_load_b64(va, ok ? base + (u32)(16 * V * (TOP - p)) : OOB);
                 return v4i{(int)nib_even((u32)x[0]), (int)nib_odd((u32)x[0]),
                            (int)nib_even((u32)x[1]), (int)nib_odd((u32)x[1])};
             }
@@ -161,8 +235,12 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) __attribute__((amdgpu_waves_per_
     constexpr int ROWS = P * V;
     constexpr int RT = (ROWS + 15) / 16;
     constexpr int CPD = 32 / BB;            // columns per B dword
-    constexpr int NMMA = LW * CPD;          // 16-column MFMA groups per wave
-    constexpr int TILE_N = 16 * NMMA;
+    // 16-column MFMA groups per B dword: one per column, two per 16-bit
+    // column (low digit at even, high digit at odd index)
+    constexpr int NV = BB == 4 ? 8 : 4;
+    constexpr int NMMA = LW * NV;           // accumulator groups per wave
+    constexpr int NCOL = LW * CPD;          // output columns per lane
+    constexpr int TILE_N = 16 * NCOL;
     constexpr int MMA_K = T::MMA_K;
     constexpr int OB = T::OB;
     constexpr int NT = 64 * TPB * SPLITK;
@@ -218,13 +296,16 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) __attribute__((amdgpu_waves_per_
     auto compute = [&](u32 (&braw)[16][LW], const v4i (&aop)[RT]) {
 #pragma unroll
         for (int d = 0; d < LW; ++d) {
-            v4i bop[CPD];
+            v4i bop[NV];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 u32 o[4];
-                if constexpr (BB == 8) {
-                    // unsigned bytes become signed digits b - 128 (OB = +128)
-                    const u32 x = SGN ? 0u : 0x80808080u;
+                if constexpr (BB != 4) {
+                    // unsigned bytes become signed digits b - 128 (OB = +128);
+                    // 16-bit B: bytes 0/2 of a dword are the low digits of its
+                    // two columns, 1/3 the high digits, so o[b] is column b/2's
+                    // digit b%2
+                    const u32 x = BB == 8 ? (SGN ? 0u : 0x80808080u) : (SGN ? 0x00800080u : 0x80808080u);
                     transpose4x4(braw[4 * q][d] ^ x, braw[4 * q + 1][d] ^ x, braw[4 * q + 2][d] ^ x,
                                  braw[4 * q + 3][d] ^ x, o);
 #pragma unroll
@@ -244,7 +325,7 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) __attribute__((amdgpu_waves_per_
 #pragma unroll
             for (int t = 0; t < RT; ++t)
 #pragma unroll
-                for (int c = 0; c < CPD; ++c) acc[t][d * CPD + c] = mfma_i8(aop[t], bop[c], acc[t][d * CPD + c]);
+                for (int c = 0; c < NV; ++c) acc[t][d * NV + c] = mfma_i8(aop[t], bop[c], acc[t][d * NV + c]);
         }
         if constexpr (OB != 0) {
 #pragma unroll
@@ -350,25 +431,34 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) __attribute__((amdgpu_waves_per_
         }
     }
 
-    int res[NMMA][HOME];
+    // sums mod 2^32 in unsigned arithmetic
+    u32 comb[NMMA][HOME];
 #pragma unroll
     for (int j = 0; j < NMMA; ++j)
 #pragma unroll
         for (int i = 0; i < HOME; ++i) {
-            int vsum = OB * rs_home[i];
+            u32 vsum = 0;
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                constexpr int dummy = 0;
-                (void)dummy;
                 int t, dg, di;
                 if constexpr (V == 8) { t = p / 2; dg = 2 * (p % 2); di = 0; }
                 else if constexpr (V == 4) { t = p / 4; dg = p % 4; di = 0; }
                 else { t = p / 8; dg = (p % 8) / 2; di = 2 * (p % 2); }
                 int x = acc[t][j][i + di];
                 if (dg != 0) x = lane_down(x, 16 * dg);
-                vsum += T::weight(p) * x;
+                vsum += (u32)T::weight(p) * (u32)x;
             }
-            res[j][i] = vsum;
+            comb[j][i] = vsum;
+        }
+    int res[NCOL][HOME];
+#pragma unroll
+    for (int c = 0; c < NCOL; ++c)
+#pragma unroll
+        for (int i = 0; i < HOME; ++i) {
+            u32 x;
+            if constexpr (BB == 16) x = (comb[2 * c + 1][i] << 8) + comb[2 * c][i];
+            else x = comb[c][i];
+            res[c][i] = (int)(x + (u32)OB * (u32)rs_home[i]);
         }
 
     const bool home = g < (V + 3) / 4;
@@ -377,8 +467,8 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) __attribute__((amdgpu_waves_per_
         __shared__ int part[(SPLITK > 1 ? SPLITK - 1 : 1) * TPB][V][TILE_N];
         if (split > 0 && home) {
 #pragma unroll
-            for (int j = 0; j < NMMA; ++j) {
-                const int col = (r * LW + j / CPD) * CPD + j % CPD;
+            for (int j = 0; j < NCOL; ++j) {
+                const int col = r * NCOL + j;
 #pragma unroll
                 for (int i = 0; i < HOME; ++i) part[(split - 1) * TPB + tile_local][4 * g + i][col] = res[j][i];
             }
@@ -389,27 +479,32 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) __attribute__((amdgpu_waves_per_
 #pragma unroll
             for (int s = 1; s < SPLITK; ++s)
 #pragma unroll
-                for (int j = 0; j < NMMA; ++j) {
-                    const int col = (r * LW + j / CPD) * CPD + j % CPD;
+                for (int j = 0; j < NCOL; ++j) {
+                    const int col = r * NCOL + j;
 #pragma unroll
-                    for (int i = 0; i < HOME; ++i) res[j][i] += part[(s - 1) * TPB + tile_local][4 * g + i][col];
+                    for (int i = 0; i < HOME; ++i)
+                        res[j][i] = (int)((u32)res[j][i] + (u32)part[(s - 1) * TPB + tile_local][4 * g + i][col]);
                 }
         }
     }
 
     if (!home || !col_ok) return;
-    // lane r owns dwords [col_dw, col_dw + LW): CPD consecutive columns each
+    // lane r owns dwords [col_dw, col_dw + LW): NCOL consecutive columns
 #pragma unroll
     for (int i = 0; i < HOME; ++i) {
         const int v = 4 * g + i;
         int *orow = args.out + (size_t)(m * V + v) * (size_t)args.n + (size_t)col_dw * CPD;
+        if constexpr (NCOL % 4 == 0) {
 #pragma unroll
-        for (int d = 0; d < LW; ++d) {
+            for (int c4 = 0; c4 < NCOL; c4 += 4) {
+                v4i o = v4i{res[c4][i], res[c4 + 1][i], res[c4 + 2][i], res[c4 + 3][i]};
+                __builtin_nontemporal_store(o, reinterpret_cast<v4i *>(orow + c4));
+            }
+        } else {
 #pragma unroll
-            for (int c4 = 0; c4 < CPD; c4 += 4) {
-                v4i o = v4i{res[d * CPD + c4][i], res[d * CPD + c4 + 1][i], res[d * CPD + c4 + 2][i],
-                            res[d * CPD + c4 + 3][i]};
-                __builtin_nontemporal_store(o, reinterpret_cast<v4i *>(orow + d * CPD + c4));
+            for (int c2 = 0; c2 < NCOL; c2 += 2) {
+                v2i o = v2i{res[c2][i], res[c2 + 1][i]};
+                __builtin_nontemporal_store(o, reinterpret_cast<v2i *>(orow + c2));
             }
         }
     }

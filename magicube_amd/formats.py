@@ -54,8 +54,13 @@ class PreparedSpmm:
     aligned_num: int
     row_offsets: np.ndarray   # int32, 2*m_vec (start, end)
     col_indices: np.ndarray   # int32, aligned_num (-1 padding; shuffled for 4-bit B)
-    values: np.ndarray        # uint8, aligned_num * V * lhs_bits / 8
+    values: np.ndarray        # uint8, aligned_num * V * container_bits(lhs_bits) / 8
     row_indices: np.ndarray   # int32, m_vec
+
+
+def container_bits(bits: int) -> int:
+    """Storage width of a lane: 12-bit lanes sit in 16-bit containers."""
+    return 16 if bits == 12 else bits
 
 
 def row_alignment(lhs_bits: int, rhs_bits: int) -> int:
@@ -87,7 +92,7 @@ def prepare_values(lhs_bits: int, rhs_bits: int, V: int, row_offsets, pairs, ali
     ro = np.ascontiguousarray(row_offsets, dtype=np.int32)
     pr = np.ascontiguousarray(pairs, dtype=np.int32)
     vals = np.ascontiguousarray(values).view(np.uint8)
-    nbytes = aligned_num * V * lhs_bits // 8
+    nbytes = aligned_num * V * container_bits(lhs_bits) // 8
     out = np.empty(max(nbytes, 1), dtype=np.uint8)
     _lib.check(_lib.lib().mc_format_prepare_values(lhs_bits, rhs_bits, V, ro.size - 1, _ptr(ro), _ptr(pr),
                                                    aligned_num, _ptr(vals), _ptr(out)),
@@ -117,8 +122,11 @@ def prepare_spmm(csr: VectorCSR, values: np.ndarray, lhs_bits: int, rhs_bits: in
 # ---- packed lanes (little-endian, lane 0 in the low bits) -----------------
 
 def pack_lanes(vals: np.ndarray, bits: int) -> np.ndarray:
-    """int lanes -> packed bytes (values truncated to ``bits``)."""
+    """int lanes -> packed bytes (values truncated to ``bits``; 12-bit lanes
+    in 16-bit containers)."""
     v = np.ascontiguousarray(vals).astype(np.int64).ravel()
+    if bits == 12:
+        return (v & 0xFFF).astype(np.uint16).view(np.uint8)
     if bits == 8:
         return (v & 0xFF).astype(np.uint8)
     if bits == 16:
@@ -135,7 +143,9 @@ def pack_lanes(vals: np.ndarray, bits: int) -> np.ndarray:
 def unpack_lanes(packed: np.ndarray, bits: int, count: int, signed: bool) -> np.ndarray:
     """packed bytes -> int32 lanes."""
     b = np.ascontiguousarray(packed).view(np.uint8).ravel()
-    if bits == 8:
+    if bits == 12:
+        out = (b[: 2 * count].view(np.uint16) & 0xFFF).astype(np.int32)
+    elif bits == 8:
         out = b[:count].astype(np.int32)
     elif bits == 16:
         out = b[: 2 * count].view(np.uint16).astype(np.int32)

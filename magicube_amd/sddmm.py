@@ -1,6 +1,6 @@
 """SDDMM operators on device tensors (host mirror of sddmm::wmmaSddmm_*).
 
-``wmmaSddmm_8b`` / ``wmmaSddmm_4b`` take the reference's arguments in the
+``wmmaSddmm_8b`` / ``wmmaSddmm_4b`` / ``wmmaSddmm_16b`` take the reference's arguments in the
 reference's order (SDDMM/SDDMM/include/wmma_sddmm.cuh:7-32) as torch tensors
 resident in HBM, plus the lane interpretation, the output layout and a stream.
 """
@@ -39,6 +39,12 @@ def wmmaSddmm_8b(m_vec, k, n, row_indices, row_offsets, col_indices, lhs_matrix,
 def wmmaSddmm_4b(m_vec, k, n, row_indices, row_offsets, col_indices, lhs_matrix, rhs_matrix, output_values,
                  vec_length, sign=SIGNED, stream=None):
     wmma_sddmm(4, m_vec, k, n, row_indices, row_offsets, col_indices, lhs_matrix, rhs_matrix, output_values,
+               vec_length, sign, OUT_REFERENCE, stream)
+
+
+def wmmaSddmm_16b(m_vec, k, n, row_indices, row_offsets, col_indices, lhs_matrix, rhs_matrix, output_values,
+                  vec_length, sign=SIGNED, stream=None):
+    wmma_sddmm(16, m_vec, k, n, row_indices, row_offsets, col_indices, lhs_matrix, rhs_matrix, output_values,
                vec_length, sign, OUT_REFERENCE, stream)
 
 

@@ -1,10 +1,11 @@
 """SpMM operators on device tensors (host mirror of spmm::wmmaSpmm_*).
 
-``wmmaSpmm_8b`` / ``wmmaSpmm_16b8b`` / ``wmmaSpmm_8b4b`` / ``wmmaSpmm_4b``
-take the reference's arguments in the reference's order
-(SpMM/SpMM/include/wmma_spmm.cuh:15-70) as torch tensors resident in HBM,
+``wmmaSpmm_8b`` / ``_12b8b`` / ``_16b8b`` / ``_16b`` / ``_8b4b`` / ``_12b4b`` /
+``_16b4b`` / ``_4b`` take the reference's arguments in the reference's order
+(SpMM/SpMM/include/wmma_spmm.cuh:15-78) as torch tensors resident in HBM,
 plus the lane interpretation and a stream, and raise ``MagicubeError`` where
-the reference returns a non-success ``cudaError_t``.
+the reference returns a non-success ``cudaError_t``.  12-bit lanes sit in
+16-bit containers.
 """
 from __future__ import annotations
 
@@ -20,7 +21,10 @@ from .formats import PreparedSpmm
 SIGNED = _lib.MC_SIGNED
 UNSIGNED = _lib.MC_UNSIGNED
 
-_PAIR_FN = {(8, 8): "mc_spmm_l8r8", (16, 8): "mc_spmm_l16r8", (8, 4): "mc_spmm_l8r4", (4, 4): "mc_spmm_l4r4"}
+_PAIR_FN = {(8, 8): "mc_spmm_l8r8", (16, 8): "mc_spmm_l16r8", (8, 4): "mc_spmm_l8r4", (4, 4): "mc_spmm_l4r4",
+            (12, 8): "mc_spmm_l12r8", (16, 16): "mc_spmm_l16r16", (12, 4): "mc_spmm_l12r4",
+            (16, 4): "mc_spmm_l16r4"}
+PAIRS = tuple(_PAIR_FN)
 
 
 def _dptr(t):
@@ -78,6 +82,30 @@ def wmmaSpmm_8b4b(m_vec, vec_length, n, k, row_indices, row_offsets, column_indi
 def wmmaSpmm_4b(m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values, rhs_matrix,
                 output_matrix, sign=SIGNED, stream=None):
     wmma_spmm(4, 4, m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values, rhs_matrix,
+              output_matrix, sign, None, stream)
+
+
+def wmmaSpmm_12b8b(m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values, rhs_matrix,
+                   output_matrix, sign=SIGNED, stream=None):
+    wmma_spmm(12, 8, m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values, rhs_matrix,
+              output_matrix, sign, None, stream)
+
+
+def wmmaSpmm_16b(m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values, rhs_matrix,
+                 output_matrix, sign=SIGNED, stream=None):
+    wmma_spmm(16, 16, m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values, rhs_matrix,
+              output_matrix, sign, None, stream)
+
+
+def wmmaSpmm_12b4b(m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values, rhs_matrix,
+                   output_matrix, sign=SIGNED, stream=None):
+    wmma_spmm(12, 4, m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values, rhs_matrix,
+              output_matrix, sign, None, stream)
+
+
+def wmmaSpmm_16b4b(m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values, rhs_matrix,
+                   output_matrix, sign=SIGNED, stream=None):
+    wmma_spmm(16, 4, m_vec, vec_length, n, k, row_indices, row_offsets, column_indices, values, rhs_matrix,
               output_matrix, sign, None, stream)
 
 

@@ -9,7 +9,8 @@ Arrays only (np.savez_compressed, no pickled objects).  Keys per case:
   spmm_L{a}R{b}_V{v}_{row_offsets,col_indices,values,rhs,out}
   sddmm_{b}b_V{v}_{row_offsets,col_indices,lhs,rhs,out}
 row_offsets / col_indices are the plain pattern (m_vec + 1 offsets); values
-and rhs hold the generator's words as bytes, lane 0 in the low bits.
+and rhs hold the generator's words as bytes, lane 0 in the low bits (12-bit
+lanes in the generator's 16-bit containers, whose bits 12..15 the checks drop).
 """
 from __future__ import annotations
 
@@ -30,6 +31,10 @@ TYPES = {
     (16, 8, 8): (8, 16, 16, 2), (16, 8, 4): (8, 16, 16, 1), (16, 8, 2): (4, 16, 16, 1),
     (8, 4, 8): (8, 8, 8, 1), (8, 4, 4): (4, 8, 8, 1), (8, 4, 2): (2, 8, 8, 1),
     (4, 4, 8): (4, 4, 4, 1), (4, 4, 4): (2, 4, 4, 1), (4, 4, 2): (1, 4, 4, 1),
+    (12, 8, 8): (8, 16, 12, 2), (12, 8, 4): (8, 16, 12, 1), (12, 8, 2): (4, 16, 12, 1),
+    (16, 16, 8): (2, 16, 16, 8), (16, 16, 4): (2, 16, 16, 4), (16, 16, 2): (2, 16, 16, 2),
+    (12, 4, 8): (8, 16, 12, 2), (12, 4, 4): (8, 16, 12, 1), (12, 4, 2): (4, 16, 12, 1),
+    (16, 4, 8): (8, 16, 16, 2), (16, 4, 4): (8, 16, 16, 1), (16, 4, 2): (4, 16, 16, 1),
 }
 M_VEC, K, N = 8, 64, 32
 
@@ -56,10 +61,10 @@ def main(path: str = OUT):
         key = f"spmm_L{pa}R{pb}_V{V}"
         data[key + "_row_offsets"] = ro
         data[key + "_col_indices"] = ci
-        data[key + "_values"] = vals.view(np.uint8)[: nnz * V * pa // 8].copy()
+        data[key + "_values"] = vals.view(np.uint8)[: nnz * V * gen_a // 8].copy()
         data[key + "_rhs"] = rhs.view(np.uint8).copy()
         data[key + "_out"] = C
-    for bits in (8, 4):
+    for bits in (8, 4, 16):
         for V in (8, 4, 2):
             ro, ci = pattern(M_VEC, N, seed=200 + bits + V)
             pairs, an = oracle.align_row_offsets(ro, 8)

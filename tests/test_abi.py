@@ -58,12 +58,13 @@ def _spmm(mclib, name, **kw):
               args["rhs"], args["out"], args["sign"], args["stream"])
 
 
-@pytest.mark.parametrize("name", ["mc_spmm_l8r8", "mc_spmm_l16r8", "mc_spmm_l8r4", "mc_spmm_l4r4"])
+@pytest.mark.parametrize("name", ["mc_spmm_l8r8", "mc_spmm_l16r8", "mc_spmm_l8r4", "mc_spmm_l4r4", "mc_spmm_l12r8",
+                                  "mc_spmm_l16r16", "mc_spmm_l12r4", "mc_spmm_l16r4"])
 def test_spmm_validation(mclib, name):
     assert _spmm(mclib, name, V=3) == _lib.MC_EINVAL_V
     assert _spmm(mclib, name, V=16) == _lib.MC_EINVAL_V
     assert _spmm(mclib, name, n=0) == _lib.MC_EINVAL_SHAPE
-    assert _spmm(mclib, name, n=6) == _lib.MC_EINVAL_SHAPE          # not a multiple of 4 / 8
+    assert _spmm(mclib, name, n=6 if not name.endswith("r16") else 7) == _lib.MC_EINVAL_SHAPE  # n % (32/bits(B))
     assert _spmm(mclib, name, k=-1) == _lib.MC_EINVAL_SHAPE
     assert _spmm(mclib, name, sign=7) == _lib.MC_EINVAL_ARG
     assert _spmm(mclib, name, ro=NULL) == _lib.MC_EINVAL_ARG
@@ -76,12 +77,15 @@ def test_spmm_validation(mclib, name):
 
 def test_spmm_run_validation(mclib):
     P = ctypes.c_void_p(16)
-    assert mclib.mc_spmm_run(12, 8, 0, -1, 4, 8, 64, 64, None, P, P, P, P, P, None) == _lib.MC_EUNSUPPORTED
+    for lhs, rhs in ((12, 16), (8, 16), (4, 8), (4, 16), (32, 8), (8, 2)):
+        assert mclib.mc_spmm_run(lhs, rhs, 0, -1, 4, 8, 64, 64, None, P, P, P, P, P, None) == _lib.MC_EUNSUPPORTED
     assert mclib.mc_spmm_run(8, 8, 0, 99, 4, 8, 64, 64, None, P, P, P, P, P, None) == _lib.MC_EINVAL_ARG
     # config 1 gathers two dwords per lane: n/4 must be even
     assert mclib.mc_spmm_run(8, 8, 0, 1, 4, 8, 68, 64, None, P, P, P, P, P, None) == _lib.MC_EINVAL_SHAPE
-    # four dwords per lane are built for 8-bit B only
+    # four dwords per lane are built for 8- and 16-bit B only; the B-stationary kernel for 8-bit B
     assert mclib.mc_spmm_run(8, 4, 0, 2, 4, 8, 512, 64, None, P, P, P, P, P, None) == _lib.MC_EUNSUPPORTED
+    assert mclib.mc_spmm_run(16, 4, 0, 2, 4, 8, 512, 64, None, P, P, P, P, P, None) == _lib.MC_EUNSUPPORTED
+    assert mclib.mc_spmm_run(16, 16, 0, 8, 4, 8, 512, 64, None, P, P, P, P, P, None) == _lib.MC_EUNSUPPORTED
 
 
 def test_tile_config_table(mclib):
@@ -109,7 +113,7 @@ def test_format_validation(mclib):
 
 def test_sddmm_validation(mclib):
     p = ctypes.c_void_p(16)
-    for fn in (mclib.mc_sddmm_l8r8, mclib.mc_sddmm_l4r4):
+    for fn in (mclib.mc_sddmm_l8r8, mclib.mc_sddmm_l4r4, mclib.mc_sddmm_l16r16):
         assert fn(4, 100, 64, None, p, p, p, p, p, 8, 0, None) == _lib.MC_EINVAL_SHAPE   # k % 64
         assert fn(4, 64, 64, None, p, p, p, p, p, 3, 0, None) == _lib.MC_EINVAL_V
         assert fn(4, 512, 64, None, p, p, p, p, p, 8, 0, None) == _lib.MC_EUNSUPPORTED   # k not built
@@ -117,4 +121,5 @@ def test_sddmm_validation(mclib):
         assert fn(4, 64, 64, None, p, p, p, p, p, 8, 5, None) == _lib.MC_EINVAL_ARG
         assert fn(0, 64, 64, None, p, p, p, p, p, 8, 0, None) == _lib.MC_OK
     assert mclib.mc_sddmm_run(8, 0, 7, 4, 64, 64, None, p, p, p, p, p, 8, None) == _lib.MC_EINVAL_ARG
-    assert mclib.mc_sddmm_run(16, 0, 0, 4, 64, 64, None, p, p, p, p, p, 8, None) == _lib.MC_EUNSUPPORTED
+    for bits in (12, 2, 32):
+        assert mclib.mc_sddmm_run(bits, 0, 0, 4, 64, 64, None, p, p, p, p, p, 8, None) == _lib.MC_EUNSUPPORTED

@@ -64,10 +64,11 @@ def test_spmm_driver_reads_smtx_and_prints_reference_lines(tmp_path):
     csr = synthetic.poisson_block_mask(256, 384, 8, 0.8, seed=3)
     path = tmp_path / "m.smtx"
     write_smtx(path, csr)
-    for preA, preB, align in ((8, 8, 16), (4, 4, 32)):
+    for preA, preB, align in ((8, 8, 16), (4, 4, 32), (12, 8, 16), (16, 16, 16), (12, 4, 32)):
         out = _header([SPMM, str(path), "64", "8", "0", "1", "1", "1", str(preA), str(preB)])
         assert out.startswith(f"Sparse matrix: {path}\n")
-        assert f"preA {preA}, preA_cut {preA}, preB {preB}, vec_length 8 \n" in out
+        # the reference prints the container width, then the value width
+        assert f"preA {16 if preA == 12 else preA}, preA_cut {preA}, preB {preB}, vec_length 8 \n" in out
         assert f"m_vec 32, dimN 64, nonzeros_vec {csr.nnz_vec}, dimk 384, mma_k_dim {align} \n" in out
         _, an = formats.align_rows(csr.row_offsets, align)
         assert f" nonzero_vec: {csr.nnz_vec} aligned_ nonzero_vec: {an}\n" in out
@@ -77,9 +78,10 @@ def test_spmm_driver_reads_smtx_and_prints_reference_lines(tmp_path):
 def test_drivers_reject_unsupported_combinations():
     _need(SPMM)
     _need(SDDMM)
-    assert "Unsupported precision and vec_length!" in _header([SPMM, "x", "64", "8", "0", "1", "1", "1", "12", "8"])
+    assert "Unsupported precision and vec_length!" in _header([SPMM, "x", "64", "8", "0", "1", "1", "1", "8", "16"])
+    assert "Unsupported precision and vec_length!" in _header([SPMM, "x", "64", "8", "0", "1", "1", "1", "12", "16"])
     assert "Unsupported Kernel" in _header([SPMM, "x", "64", "8", "2", "1", "1", "1", "8", "8"])
-    assert "Unsupported precision!" in _header([SDDMM, "x", "64", "8", "1", "1", "1", "16", "16"])
+    assert "Unsupported precision!" in _header([SDDMM, "x", "64", "8", "1", "1", "1", "16", "8"])
     assert "Unsupported vec_length!" in _header([SDDMM, "x", "64", "3", "1", "1", "1", "8", "8"])
 
 

@@ -13,7 +13,8 @@ PKG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 
 
 
 @pytest.mark.parametrize("preA,preB,V", [(8, 8, 8), (8, 8, 4), (8, 8, 2), (16, 8, 8), (16, 8, 2), (8, 4, 8),
-                                         (8, 4, 4), (4, 4, 8), (4, 4, 2)])
+                                         (8, 4, 4), (4, 4, 8), (4, 4, 2), (12, 8, 8), (12, 8, 2), (16, 16, 8),
+                                         (16, 16, 4), (12, 4, 8), (12, 4, 2), (16, 4, 8), (16, 4, 4)])
 def test_spmm_driver_passes(preA, preB, V):
     cmd = [os.path.join(PKG, "spmm_benchmark"), "synthetic:256,512,0.8,11", "128", str(V), "0", "1", "1", "1",
            str(preA), str(preB)]
@@ -25,7 +26,7 @@ def test_spmm_driver_passes(preA, preB, V):
     assert re.search(r"performance TOP/s: [0-9.e+-]+", r.stdout)
 
 
-@pytest.mark.parametrize("bits,K,V", [(8, 64, 8), (8, 128, 4), (4, 64, 8), (4, 256, 2)])
+@pytest.mark.parametrize("bits,K,V", [(8, 64, 8), (8, 128, 4), (4, 64, 8), (4, 256, 2), (16, 64, 8), (16, 256, 2)])
 def test_sddmm_driver_passes(bits, K, V):
     cmd = [os.path.join(PKG, "sddmm_benchmark"), "synthetic:256,384,0.8,13", str(K), str(V), "1", "1", "1",
            str(bits), str(bits)]

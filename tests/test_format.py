@@ -21,14 +21,18 @@ def test_align_rows_and_cols(orc, align):
 
 
 @pytest.mark.parametrize("V", [2, 4, 8])
-@pytest.mark.parametrize("preA,preB", [(8, 8), (16, 8), (8, 4), (4, 4)])
+@pytest.mark.parametrize("preA,preB", [(8, 8), (16, 8), (8, 4), (4, 4), (12, 8), (16, 16), (12, 4), (16, 4)])
 def test_prepare_values_matches_reference(orc, V, preA, preB):
     m_vec, K = 24, 160
     tb, pa, cut, scale = REF_TABLE[(preA, preB, V)]
     csr = small_mask(m_vec, K, V, 0.25, seed=preA * 10 + V)
     nnz = csr.nnz_vec
-    lanes = synthetic.random_lanes(nnz * V, preA, True, seed=5)
-    packed = formats.pack_lanes(lanes, preA)
+    if preA == 12:
+        # full 16-bit containers: both preparations keep bits 0..11 only
+        packed = formats.pack_lanes(synthetic.random_lanes(nnz * V, 16, True, seed=5), 16)
+    else:
+        lanes = synthetic.random_lanes(nnz * V, preA, True, seed=5)
+        packed = formats.pack_lanes(lanes, preA)
     align = formats.row_alignment(preA, preB)
     pairs, an = formats.align_rows(csr.row_offsets, align)
     mine = formats.prepare_values(preA, preB, V, csr.row_offsets, pairs, an, packed)
@@ -46,6 +50,16 @@ def test_sorted_rows(orc):
     lens = np.diff(csr.row_offsets)[mine]
     assert (np.diff(lens) <= 0).all()
     assert np.array_equal(mine, orc.sorted_row_swizzle(csr.row_offsets))
+
+
+def test_pack_unpack_12_bit_containers():
+    lanes = synthetic.random_lanes(777, 12, True, seed=12)
+    packed = formats.pack_lanes(lanes, 12)
+    assert packed.size == 2 * lanes.size
+    assert (packed.view(np.uint16) < 4096).all()
+    assert np.array_equal(formats.unpack_lanes(packed, 12, lanes.size, True), lanes)
+    u = synthetic.random_lanes(777, 12, False, seed=13)
+    assert np.array_equal(formats.unpack_lanes(formats.pack_lanes(u, 12), 12, u.size, False), u)
 
 
 @pytest.mark.parametrize("bits", [4, 8, 16])

@@ -18,7 +18,7 @@ def test_spmm_on_golden_fixtures():
     d = np.load(GOLDEN, allow_pickle=False)
     m_vec, K, N = (int(x) for x in d["dims"])
     keys = sorted({k.rsplit("_", 1)[0] for k in d.files if k.startswith("spmm_") and k.endswith("_out")})
-    assert len(keys) == 12
+    assert len(keys) == 24
     for key in keys:
         pa, pb = (int(x) for x in key.split("_")[1][1:].split("R"))
         V = int(key.split("_V")[1])
@@ -35,7 +35,7 @@ def test_sddmm_on_golden_fixtures():
     d = np.load(GOLDEN, allow_pickle=False)
     m_vec, K, N = (int(x) for x in d["dims"])
     keys = sorted({k.rsplit("_", 1)[0] for k in d.files if k.startswith("sddmm_") and k.endswith("_out")})
-    assert len(keys) == 6
+    assert len(keys) == 9
     for key in keys:
         bits = int(key.split("_")[1][:-1])
         V = int(key.split("_V")[1])

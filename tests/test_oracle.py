@@ -263,7 +263,7 @@ def test_published_counter_invariant(orc, V):
     assert round(2 * 176947 * 8 * 512 / 1e9, 5) == 1.44955
 
 
-@pytest.mark.parametrize("preA,V", [(8, 8), (8, 2), (4, 8)])
+@pytest.mark.parametrize("preA,V", [(8, 8), (8, 2), (4, 8), (16, 8), (16, 2)])
 def test_sddmm_ref_check_equals_plain_unsigned(orc, preA, V):
     m_vec, N, K = 12, 48, 64
     csr = small_mask(m_vec, N, V, 0.3, seed=13)
@@ -303,14 +303,19 @@ def read_prepared(prep, lhs_bits, rhs_bits, V, pairs, aligned_num):
     mma_k = 32 if (lhs_bits == 4 or rhs_bits == 4) else 16
     out = np.zeros((aligned_num, V), dtype=np.int64)
     b = prep.astype(np.int64)
-    gb = mma_k * V * lhs_bits // 8
+    cb = 16 if lhs_bits == 12 else lhs_bits          # 12-bit lanes use 16-bit containers
+    gb = mma_k * V * cb // 8
     for s in range(aligned_num):
         G, u = divmod(s, mma_k)
         base = G * gb
         for v in range(V):
             if lhs_bits == 8 and rhs_bits == 8:
                 out[s, v] = b[base + 16 * v + u]
-            elif lhs_bits == 16:
+            elif lhs_bits >= 12 and rhs_bits == 4:
+                # nibble planes q (bits 4q..4q+3) at 16*V*q
+                out[s, v] = sum(((b[base + 16 * V * q + 16 * v + u // 2] >> (4 * (u & 1))) & 0xF) << (4 * q)
+                                for q in range(lhs_bits // 4))
+            elif lhs_bits >= 12:
                 out[s, v] = b[base + 16 * v + u] | (b[base + 16 * V + 16 * v + u] << 8)
             elif lhs_bits == 8:
                 lo = (b[base + 16 * v + u // 2] >> (4 * (u & 1))) & 0xF
@@ -322,14 +327,19 @@ def read_prepared(prep, lhs_bits, rhs_bits, V, pairs, aligned_num):
 
 
 @pytest.mark.parametrize("V", [2, 4, 8])
-@pytest.mark.parametrize("preA,preB", [(8, 8), (16, 8), (8, 4), (4, 4)])
+@pytest.mark.parametrize("preA,preB", [(8, 8), (16, 8), (8, 4), (4, 4), (12, 8), (16, 16), (12, 4), (16, 4)])
 def test_prepared_image_encodes_values(orc, V, preA, preB):
     m_vec, K = 10, 80
     tb, pa, cut, scale = REF_TABLE[(preA, preB, V)]
     csr = small_mask(m_vec, K, V, 0.3, seed=V + preA)
     nnz = csr.nnz_vec
     lanes = synthetic.random_lanes(nnz * V, preA, False, seed=3)
-    vals = orc.pack_lanes(lanes, preA)
+    if preA == 12:
+        # 16-bit containers with junk in bits 12..15, which the layout drops
+        junk = synthetic.random_lanes(nnz * V, 4, False, seed=4)
+        vals = orc.pack_lanes(lanes | (junk << 12), 16)
+    else:
+        vals = orc.pack_lanes(lanes, preA)
     vals = np.concatenate([vals, np.zeros((-vals.size) % tb, np.uint8)]).view({1: np.int8, 2: np.int16,
                                                                                  4: np.int32, 8: np.int64}[tb])
     mma_k = 32 if (preA == 4 or preB == 4) else 16

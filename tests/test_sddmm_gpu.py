@@ -51,14 +51,14 @@ def expected(orc, dm, csr, lhs, rhs, K, V):
 @pytest.mark.parametrize("K", [64, 128, 256])
 @pytest.mark.parametrize("signed", [True, False])
 @pytest.mark.parametrize("V", [2, 4, 8])
-@pytest.mark.parametrize("bits", [8, 4])
+@pytest.mark.parametrize("bits", [8, 4, 16])
 def test_sddmm_small_exact(orc, bits, V, signed, K):
     csr, lhs, rhs = make(bits, V, signed, M=24 * V, N=300, K=K, sparsity=0.8, seed=V * 7 + K + bits)
     dm, got = run(bits, V, signed, csr, lhs, rhs, K)
     assert np.array_equal(got, expected(orc, dm, csr, lhs, rhs, K, V))
 
 
-@pytest.mark.parametrize("bits", [8, 4])
+@pytest.mark.parametrize("bits", [8, 4, 16])
 def test_sddmm_vector_layout(orc, bits):
     V, K = 8, 128
     csr, lhs, rhs = make(bits, V, True, M=256, N=512, K=K, sparsity=0.9, seed=3)
@@ -71,7 +71,7 @@ def test_sddmm_vector_layout(orc, bits):
                 assert vec[j * V + v] == ref[(j // 8) * 8 * V + 8 * v + j % 8]
 
 
-@pytest.mark.parametrize("bits", [8, 4])
+@pytest.mark.parametrize("bits", [8, 4, 16])
 def test_sddmm_reference_generator_unsigned(orc, bits):
     # MakeDenseMatrix<int> operands, checked against the literal host loop
     V, M, N, K = 8, 512, 512, 128
@@ -90,7 +90,7 @@ def test_sddmm_reference_generator_unsigned(orc, bits):
 
 @pytest.mark.parametrize("sparsity", [0.9, 0.95])
 @pytest.mark.parametrize("K", [64, 128])
-@pytest.mark.parametrize("bits", [8, 4])
+@pytest.mark.parametrize("bits", [8, 4, 16])
 def test_sddmm_full_size_exact(orc, bits, K, sparsity):
     # Q 4096 x K, K-matrix 4096 x K, V=8 mask: the SDDMM workload
     V = 8
@@ -103,3 +103,15 @@ def test_sddmm_full_size_exact(orc, bits, K, sparsity):
     for v in range(V):
         real[(idx // 8) * 8 * V + 8 * v + idx % 8] = True
     assert np.array_equal(got[real], exp_plain[real])
+
+
+@pytest.mark.parametrize("signed", [True, False])
+def test_sddmm_16b_extreme_values(orc, signed):
+    # every lane at the range ends: the offset terms (K * OA * OB) wrap mod 2^32
+    V, K = 8, 256
+    csr, lhs, rhs = make(16, V, signed, M=64, N=96, K=K, sparsity=0.5, seed=21)
+    lo, hi = (-(1 << 15), (1 << 15) - 1) if signed else (0, (1 << 16) - 1)
+    lhs = np.where(lhs % 2 == 0, lo, hi).astype(np.int32)
+    rhs = np.where(rhs % 3 == 0, lo, hi).astype(np.int32)
+    dm, got = run(16, V, signed, csr, lhs, rhs, K)
+    assert np.array_equal(got, expected(orc, dm, csr, lhs, rhs, K, V))

@@ -16,7 +16,12 @@ from tests.spmm_cases import checksum_expected, checksum_got, make_case
 
 pytestmark = pytest.mark.gpu
 
-PAIRS = [(8, 8), (16, 8), (8, 4), (4, 4)]
+PAIRS = [(8, 8), (16, 8), (8, 4), (4, 4), (12, 8), (16, 16), (12, 4), (16, 4)]
+BASE_PAIRS = PAIRS[:4]
+
+
+def pair_id(p):
+    return f"L{p[0]}R{p[1]}" if isinstance(p, tuple) else None
 
 
 def run_case(case, config=None, row_order="sorted", prefill=0x5A5A5A5A, stream=None):
@@ -42,7 +47,7 @@ def oracle_result(orc, case):
 
 @pytest.mark.parametrize("signed", [True, False])
 @pytest.mark.parametrize("V", [2, 4, 8])
-@pytest.mark.parametrize("pair", PAIRS)
+@pytest.mark.parametrize("pair", PAIRS, ids=pair_id)
 def test_spmm_small_exact(orc, pair, V, signed):
     case = make_case(*pair, V, signed, M=96 * V // 2, K=384, N=256, sparsity=0.8, seed=11 * V + pair[0])
     got = run_case(case)
@@ -53,18 +58,18 @@ GATHER_CONFIGS = [0, 1, 2, 3, 4, 5, 6, 7, 12]
 
 
 @pytest.mark.parametrize("config", GATHER_CONFIGS)
-@pytest.mark.parametrize("pair", PAIRS)
+@pytest.mark.parametrize("pair", PAIRS, ids=pair_id)
 def test_spmm_every_tile_config(orc, pair, config):
     assert spmm.tile_configs()[config][4] == 0
     if config in (2, 3, 6) and pair[1] == 4:
-        pytest.skip("four dwords per lane are built for 8-bit B only")
+        pytest.skip("four dwords per lane are built for 8- and 16-bit B only")
     for V in (2, 8):
         case = make_case(*pair, V, True, M=64 * V, K=512, N=512, sparsity=0.85, seed=config * 7 + V)
         got = run_case(case, config=config)
         assert np.array_equal(got, oracle_result(orc, case)), f"V={V}"
 
 
-@pytest.mark.parametrize("pair", PAIRS)
+@pytest.mark.parametrize("pair", PAIRS, ids=pair_id)
 def test_spmm_ragged_n_and_empty_rows(orc, pair):
     # N not a multiple of the tile; rows with 0, 1 and many nonzeros
     cpd = 32 // pair[1]
@@ -136,10 +141,11 @@ def test_headline_exact(orc, signed):
     assert np.array_equal(run_case(case), oracle_result(orc, case))
 
 
-SWEEP = [(pair, V, s, N) for pair in PAIRS for V in (2, 4, 8) for s in (0.7, 0.9, 0.98) for N in (128, 256, 512)]
+SWEEP = ([(pair, V, s, N) for pair in BASE_PAIRS for V in (2, 4, 8) for s in (0.7, 0.9, 0.98) for N in (128, 256, 512)]
+         + [(pair, V, 0.9, 256) for pair in PAIRS[4:] for V in (2, 4, 8)])
 
 
-@pytest.mark.parametrize("pair,V,s,N", SWEEP)
+@pytest.mark.parametrize("pair,V,s,N", SWEEP, ids=lambda x: pair_id(x) if isinstance(x, tuple) else str(x))
 def test_sweep_checksums(pair, V, s, N):
     case = make_case(*pair, V, True, M=4096, K=4096, N=N, sparsity=s, seed=int(s * 100) + V + N)
     got = run_case(case)
@@ -147,3 +153,20 @@ def test_sweep_checksums(pair, V, s, N):
     for _ in range(2):
         w = rng.integers(0, 1 << 10, size=N)
         assert np.array_equal(checksum_got(got, w), checksum_expected(case, w))
+
+
+@pytest.mark.parametrize("signed", [True, False])
+@pytest.mark.parametrize("pair", PAIRS[4:], ids=pair_id)
+def test_spmm_extreme_values(orc, pair, signed):
+    # lanes at the ends of their ranges: digit offsets and weights at their limits
+    case = make_case(*pair, 8, signed, M=256, K=512, N=128, sparsity=0.7, seed=99)
+    la, lb = pair
+    def ends(x, bits):
+        lo, hi = (-(1 << (bits - 1)), (1 << (bits - 1)) - 1) if signed else (0, (1 << bits) - 1)
+        return np.where(x % 2 == 0, lo, hi).astype(np.int32)
+    case.a = ends(case.a, la)
+    case.b = ends(case.b, lb)
+    case.prep = formats.prepare_spmm(case.csr, formats.pack_lanes(case.a, la), la, lb)
+    rhs = formats.pack_lanes(case.b, lb)
+    case.rhs_packed = np.concatenate([rhs, np.zeros((-rhs.size) % 4, np.uint8)]).view(np.int32)
+    assert np.array_equal(run_case(case), oracle_result(orc, case))
