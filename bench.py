@@ -34,6 +34,27 @@ HBM_PEAK_GBS = 8000.0
 MFMA_INT8_PEAK_TOPS = 5000.0
 REFERENCE_A100_TOPS = 16.67   # plot/examples/spmm_pres.txt:531-532 (DLMC 256x2304 rows, V=8, s=0.9, N=512)
 
+# The reference's precision sweep on A100 at its published shape (DLMC rn50
+# random_pruning 0.9 bottleneck_2_block_group3_5_1: 256 vector rows, K = 2304,
+# N = 512, unsigned lanes): (lhs, rhs, V) -> (TOPS, plot/examples/spmm_pres.txt line).
+# L12-R8 has no published figure.
+PUBLISHED_SPMM_PRES = {
+    (16, 16, 2): (2.26475, 110), (16, 16, 4): (4.46368, 121), (16, 16, 8): (10.2298, 132),
+    (16, 8, 2): (3.64078, 310), (16, 8, 4): (7.03666, 321), (16, 8, 8): (15.7426, 332),
+    (8, 8, 2): (4.34606, 510), (8, 8, 4): (8.38464, 521), (8, 8, 8): (16.6716, 532),
+    (16, 4, 2): (4.29697, 710), (16, 4, 4): (8.23018, 721), (16, 4, 8): (17.4599, 732),
+    (12, 4, 2): (4.25792, 910), (12, 4, 4): (8.47903, 921), (12, 4, 8): (20.1014, 932),
+    (8, 4, 2): (4.63334, 1110), (8, 4, 4): (8.65352, 1121), (8, 4, 8): (20.4712, 1132),
+    (4, 4, 2): (5.95672, 1310), (4, 4, 4): (11.6777, 1321), (4, 4, 8): (21.0431, 1332),
+}
+# SDDMM on A100 at the same matrix, N = 2304, K = 512 (basic kernel):
+# (bits, V) -> (TOPS, plot/examples/sddmm_abl_study.txt line)
+PUBLISHED_SDDMM_K512 = {
+    (16, 2): (2.23103, 58), (16, 4): (4.58991, 64), (16, 8): (9.71052, 70),
+    (8, 2): (3.32038, 166), (8, 4): (6.5755, 172), (8, 8): (12.6419, 178),
+    (4, 2): (4.44799, 284), (4, 4): (8.83131, 291), (4, 8): (17.2533, 298),
+}
+
 
 def make_workload(M, K, N, V, sparsity, seed, lhs_bits=8, rhs_bits=8, signed=True):
     from magicube_amd import formats, synthetic
@@ -47,7 +68,8 @@ def make_workload(M, K, N, V, sparsity, seed, lhs_bits=8, rhs_bits=8, signed=Tru
 
 
 def algorithmic_bytes(nnz_vec, m_vec, M, K, N, V, lhs_bits=8, rhs_bits=8):
-    return nnz_vec * (V * lhs_bits // 8 + 4) + 4 * (m_vec + 1) + K * N * rhs_bits // 8 + 4 * M * N
+    cb = 16 if lhs_bits == 12 else lhs_bits       # 12-bit lanes are stored in 16-bit containers
+    return nnz_vec * (V * cb // 8 + 4) + 4 * (m_vec + 1) + K * N * rhs_bits // 8 + 4 * M * N
 
 
 def checksum_ok(C, csr, a, b, K, N, V, seed=0):
@@ -182,6 +204,93 @@ def sddmm_bench(steps, warmup, seed=31):
                             "hbm_frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "verified": ok})
     return {"workload": "SDDMM M=N=4096, V=8, uniform mask, signed int8/int4, reference output layout",
             "cases": out}
+
+
+def precision_bench(steps, warmup, seed=17):
+    """Every precision pair and V at the reference's published sweep shape
+    (256 vector rows, K = 2304, N = 512, sparsity 0.9, unsigned lanes as the
+    reference's MMAs), beside the A100 figure; checksum-verified; default
+    tile configuration."""
+    import torch
+    from magicube_amd import spmm
+    stream = torch.cuda.Stream()
+    K, N, s = 2304, 512, 0.9
+    cases = []
+    for (la, lb) in spmm.PAIRS:
+        for V in (2, 4, 8):
+            M = 256 * V
+            csr, a, b, _, prep, rhs_np = make_workload(M, K, N, V, s, seed, la, lb, signed=False)
+            dev = spmm.DeviceSpmm.upload(prep)
+            rhs = torch.from_numpy(rhs_np).cuda()
+            out = torch.empty((M, N), dtype=torch.int32, device="cuda")
+
+            def run():
+                dev.matmul(rhs, N, out=out, sign=spmm.UNSIGNED, stream=stream)
+
+            with torch.cuda.stream(stream):
+                run()
+            torch.cuda.synchronize()
+            ok = checksum_ok(out.cpu().numpy(), csr, a, b, K, N, V)
+            ms, _ = time_gpu(run, steps, warmup, stream)
+            nnz = csr.nnz_vec
+            tops = 2.0 * nnz * V * N / (ms * 1e-3) / 1e12
+            alg = algorithmic_bytes(nnz, csr.m_vec, M, K, N, V, la, lb)
+            pub = PUBLISHED_SPMM_PRES.get((la, lb, V))
+            cases.append({"pair": f"L{la}-R{lb}", "V": V, "nnz_vec": nnz, "us": ms * 1e3, "tops": tops,
+                          "hbm_frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "a100_tops": pub[0] if pub else None,
+                          "ratio_vs_a100": tops / pub[0] if pub else None, "verified": ok})
+    return {"workload": "SpMM m_vec=256, K=2304, N=512, s=0.9 (Poisson vector-sparse), unsigned, every pair x V",
+            "a100_source": "plot/examples/spmm_pres.txt", "cases": cases}
+
+
+def sddmm_published_bench(steps, warmup, seed=43):
+    """SDDMM at the reference's published configuration (256 vector rows,
+    N = 2304, K = 512, sparsity 0.9, unsigned lanes), every width and V,
+    beside the A100 figure; exact check on sampled vector rows."""
+    import torch
+    from magicube_amd import formats, sddmm, synthetic
+    stream = torch.cuda.Stream()
+    N, K, s = 2304, 512, 0.9
+    cases = []
+    for bits in (16, 8, 4):
+        for V in (2, 4, 8):
+            M = 256 * V
+            csr = synthetic.poisson_block_mask(M, N, V, s, seed)
+            mask = sddmm.DeviceMask.upload(csr)
+            a = synthetic.random_lanes(M * K, bits, False, seed + 1)
+            b = synthetic.random_lanes(N * K, bits, False, seed + 2)
+            lhs = torch.from_numpy(formats.pack_lanes(a, bits).view(np.int32)).cuda()
+            rhs = torch.from_numpy(formats.pack_lanes(b, bits).view(np.int32)).cuda()
+            res = torch.zeros(mask.output_size(), dtype=torch.int32, device="cuda")
+
+            def run():
+                mask.sddmm(bits, lhs, rhs, K, out=res, sign=sddmm.UNSIGNED, stream=stream)
+
+            with torch.cuda.stream(stream):
+                run()
+            torch.cuda.synchronize()
+            got = res.cpu().numpy()
+            A = a.astype(np.int64).reshape(M, K)
+            B = b.astype(np.int64).reshape(N, K)
+            ok = True
+            for i in np.random.default_rng(seed).choice(csr.m_vec, 8, replace=False):
+                s0, e0 = mask.pairs[2 * i], mask.pairs[2 * i + 1]
+                cols = csr.col_indices[csr.row_offsets[i]:csr.row_offsets[i + 1]]
+                exp = (A[i * V:(i + 1) * V] @ B[cols].T) & 0xFFFFFFFF
+                j = np.arange(s0, e0)
+                idx = (j[None, :] // 8) * 8 * V + 8 * np.arange(V)[:, None] + (j[None, :] % 8)
+                ok &= bool(np.array_equal(got[idx].astype(np.int64) & 0xFFFFFFFF, exp))
+            ms, _ = time_gpu(run, steps, warmup, stream)
+            nnz = csr.nnz_vec
+            tops = 2.0 * nnz * V * K / (ms * 1e-3) / 1e12
+            alg = M * K * bits // 8 + N * K * bits // 8 + 4 * nnz + 4 * (csr.m_vec + 1) + 4 * V * nnz
+            pub = PUBLISHED_SDDMM_K512[(bits, V)]
+            cases.append({"bits": bits, "V": V, "nnz_vec": nnz, "us": ms * 1e3, "tops": tops,
+                          "hbm_frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "a100_tops": pub[0],
+                          "ratio_vs_a100": tops / pub[0], "verified": ok})
+    return {"workload": "SDDMM m_vec=256, N=2304, K=512, s=0.9 (Poisson vector-sparse), unsigned, 16/8/4-bit x V",
+            "a100_source": "plot/examples/sddmm_abl_study.txt", "cases": cases}
 
 
 def dense_attention_fp64_torch(q, k, v, mask, V):
@@ -345,6 +454,8 @@ def main():
     ap.add_argument("--no-attention", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-sddmm", action="store_true")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the every-pair sweep and the published-shape SDDMM")
     ap.add_argument("--config", type=int, default=None, help="explicit SpMM tile config index")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 64 MiB device copy (counter calibration for rocprofv3 --pmc)")
@@ -424,6 +535,12 @@ def main():
 
     if not args.no_sddmm:
         result["sddmm"] = sddmm_bench(max(1, args.steps // 2), max(1, args.warmup // 2))
+
+    if not args.no_precision:
+        result["precision"] = precision_bench(max(1, args.steps // 2), max(1, args.warmup // 2))
+        if not args.no_sddmm:
+            result["sddmm_published_shape"] = sddmm_published_bench(max(1, args.steps // 2),
+                                                                    max(1, args.warmup // 2))
 
     if not args.no_attention:
         result["attention"] = attention_bench(max(1, args.steps // 10), max(1, args.warmup // 4))

@@ -139,8 +139,9 @@ mc_status mc_spmm_set_tile_config(int lhs_bits, int rhs_bits, int vec_length, in
  * launchers SDDMM/SDDMM/src/wmma_sddmm.cu:376-514): same arguments in the
  * same order (note k before n), plus the lane interpretation and a stream.
  *   out[j][v] = sum_k lhs[(i*V+v)][k] * rhs[col_j][k]
- * lhs is (m_vec*vec_length) x k, rhs n x k, both row-major packed; k is 64,
- * 128 or 256; row starts are multiples of 8.  Every slot of
+ * lhs is (m_vec*vec_length) x k, rhs n x k, both row-major packed; k is a
+ * multiple of 64 (as the reference's Tile_K loop); row starts are multiples
+ * of 8.  Every slot of
  * [start, start + roundup8(end - start)) is written, padding slots with 0, in
  * the reference layout [(j/8)][v][j%8] (SDDMM/SDDMM/sddmm_benchmark.cpp:55). */
 mc_status mc_sddmm_l8r8(int m_vec, int k, int n, const int *row_indices, const int *row_offsets,

@@ -21,6 +21,7 @@ mc_status sddmm_launch_k(int nkc, const SddmmArgs &a, hipStream_t s) {
     case 1: return sddmm_launch<V, BITS, SGN, 1, REF>(a, s);
     case 2: return sddmm_launch<V, BITS, SGN, 2, REF>(a, s);
     case 4: return sddmm_launch<V, BITS, SGN, 4, REF>(a, s);
+    case 8: return sddmm_launch<V, BITS, SGN, 8, REF>(a, s);
     default: return MC_EUNSUPPORTED;
     }
 }
@@ -49,8 +50,10 @@ mc_status sddmm_entry(int bits, mc_signedness sign, int layout, int m_vec, int k
     if (layout != MC_SDDMM_OUT_REFERENCE && layout != MC_SDDMM_OUT_VECTOR) return MC_EINVAL_ARG;
     if (!valid_v(vec_length)) return MC_EINVAL_V;
     if (m_vec < 0 || n <= 0 || k <= 0 || k % 64 != 0) return MC_EINVAL_SHAPE;
-    const int nkc = k / 64;
-    if (nkc != 1 && nkc != 2 && nkc != 4) return MC_EUNSUPPORTED;
+    // 64-deep chunks held per pass: the smallest of 1, 2, 4, 8 that covers k,
+    // else 8 with several passes
+    const int chunks = k / 64;
+    const int nkc = chunks <= 1 ? 1 : chunks <= 2 ? 2 : chunks <= 4 ? 4 : 8;
     const long long lhs_bytes = (long long)m_vec * vec_length * k * bits / 8;
     const long long rhs_bytes = (long long)n * k * bits / 8;
     if (lhs_bytes >= (1LL << 30) || rhs_bytes >= (1LL << 30)) return MC_EINVAL_SHAPE;

@@ -20,6 +20,10 @@
 // planes stack as MFMA rows (high plane rows 0..V-1); the two rhs digits are
 // two accumulators.  Results are exact modulo 2^32.
 //
+// K is any multiple of 64 (the reference's Tile_K loop,
+// SDDMM/SDDMM/src/wmma_sddmm.cu:77,174,261): at most NKC 64-deep chunks per
+// pass, later passes accumulating into the output.
+//
 // Output slots: every slot of [start, roundup8(end)) is written (padding
 // slots get 0), in the reference layout [(j/8)][v][j%8] (OUT_REF) or
 // vector-major [j][v].
@@ -119,144 +123,160 @@ __global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
     const __amdgpu_buffer_rsrc_t rl = make_rsrc(args.lhs, args.lhs_bytes);
     const __amdgpu_buffer_rsrc_t rr = make_rsrc(args.rhs, args.rhs_bytes);
 
-    // A operands (lhs rows of this vector row) for every k chunk, and their
-    // digit row sums for the offset correction.
+    // K runs in passes of up to NKC 64-deep chunks; every pass holds its A
+    // operands in registers and adds its partial sums (exact mod 2^32, the
+    // offset terms included) to the output slots the first pass wrote.
+    const int chunks = args.k >> 6;
+    const int passes = (chunks + NKC - 1) / NKC;
     const bool a_live = r < ROWS;
     const int a_p = r / V, a_v = r % V;
-    v4i aop[NKC];
-    int rsum = 0;
-#pragma unroll
-    for (int kc = 0; kc < NKC; ++kc) {
-        const u32 off = a_live ? (u32)(m * V + a_v) * row_bytes + (u32)(kc * KB + 16 * g * BITS / 8) : 0x80000000u;
-        v4i x;
-        if constexpr (W16) {
-            v4i hi, lo;
-            load_k16_w16(rl, off, XHI, XLO, hi, lo);
-            // rows past 2V load nothing (offset out of range) but keep the
-            // XOR masks: zero them so every dead row sums to 0
-            x = a_live ? (a_p == 0 ? hi : lo) : v4i{0, 0, 0, 0};
-        } else {
-            x = load_k16<BITS>(rl, off);
-        }
-        if constexpr (BITS == 8) {
-            if constexpr (!SGN) {
-                const int sh = a_p == 0 ? 4 : 0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) x[q] = (int)(((u32)x[q] >> sh) & 0x0F0F0F0Fu);
-            }
-        } else if constexpr (BITS == 4) {
-            x = expand_nibbles(x, SGN, 0u);
-        }
-        aop[kc] = x;
-        if constexpr (OB != 0) rsum += sum_bytes(x);
-    }
-    int rs_home[HOME];
-#pragma unroll
-    for (int i = 0; i < HOME; ++i) rs_home[i] = 0;
-    if constexpr (OB != 0) {
-        int s = rsum;
-        s += lane_read(s, lane ^ 16);
-        s += lane_read(s, lane ^ 32);   // full row sum of MFMA row r
-#pragma unroll
-        for (int i = 0; i < HOME; ++i) {
-            const int v = 4 * g + i;
-#pragma unroll
-            for (int p = 0; p < P; ++p) rs_home[i] += D::weight(p) * lane_read(s, (p * V + v) & 15);
-        }
-    }
+    for (int ps = 0; ps < passes; ++ps) {
+        const int kc0 = ps * NKC;
+        const int kn = min(NKC, chunks - kc0);   // chunks of this pass (uniform)
 
-    const int ntiles_total = (wend - start + 15) >> 4;
-    for (int c0 = start; c0 < wend; c0 += kSddmmStage) {
-        const int cn = min(wend - c0, kSddmmStage);
-        __syncthreads();
-        for (int i = threadIdx.x; i < ((cn + 15) & ~15); i += 64 * W) {
-            u32 o = kSddmmOob;
-            if (c0 + i < end) {
-                const int idx = args.col_indices[c0 + i];
-                if (idx >= 0) o = (u32)idx * row_bytes;
+        // A operands (lhs rows of this vector row) for the pass's chunks, and
+        // their digit row sums for the offset correction.
+        v4i aop[NKC];
+        int rsum = 0;
+#pragma unroll
+        for (int kc = 0; kc < NKC; ++kc) {
+            const bool live = a_live && kc < kn;
+            const u32 off =
+                live ? (u32)(m * V + a_v) * row_bytes + (u32)((kc0 + kc) * KB + 16 * g * BITS / 8) : 0x80000000u;
+            v4i x;
+            if constexpr (W16) {
+                v4i hi, lo;
+                load_k16_w16(rl, off, XHI, XLO, hi, lo);
+                // dead rows and chunks load nothing (offset out of range) but
+                // keep the XOR masks: zero them so they sum to 0
+                x = live ? (a_p == 0 ? hi : lo) : v4i{0, 0, 0, 0};
+            } else {
+                x = load_k16<BITS>(rl, off);
             }
-            s_off[i] = o;
+            if constexpr (BITS == 8) {
+                if constexpr (!SGN) {
+                    const int sh = a_p == 0 ? 4 : 0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) x[q] = (int)(((u32)x[q] >> sh) & 0x0F0F0F0Fu);
+                }
+            } else if constexpr (BITS == 4) {
+                x = expand_nibbles(x, SGN, 0u);
+            }
+            aop[kc] = x;
+            if constexpr (OB != 0) rsum += sum_bytes(x);
         }
-        __syncthreads();
-        const int ctiles = (cn + 15) >> 4;
-        (void)ntiles_total;
-        // wave w handles tile groups w, w + W, ... of TPW tiles each
-        for (int tg = wave * TPW; tg < ctiles; tg += W * TPW) {
-            constexpr int BD = W16 ? 2 : 1;   // rhs digits (16-bit: low, high)
-            v4i bop[TPW][NKC][BD];
-            int bsum[TPW];                    // sum of rhs' over this lane's k (16-bit)
+        int rs_home[HOME];
 #pragma unroll
-            for (int t = 0; t < TPW; ++t) {
-                const int tile = min(tg + t, ctiles - 1);
-                const u32 add = (tg + t < ctiles) ? 0u : kSddmmOob;
-                const u32 o = s_off[tile * 16 + r] + add;
-                bsum[t] = 0;
+        for (int i = 0; i < HOME; ++i) rs_home[i] = 0;
+        if constexpr (OB != 0) {
+            int s = rsum;
+            s += lane_read(s, lane ^ 16);
+            s += lane_read(s, lane ^ 32);   // full row sum of MFMA row r
 #pragma unroll
-                for (int kc = 0; kc < NKC; ++kc) {
-                    if constexpr (W16) {
-                        v4i hi, lo;
-                        load_k16_w16(rr, o + (u32)(kc * KB + 32 * g), XHI, XLO, hi, lo);
-                        bop[t][kc][0] = lo;
-                        bop[t][kc][BD - 1] = hi;
-                        bsum[t] += 256 * sum_bytes(hi) + sum_bytes(lo);
-                        continue;
-                    }
-                    v4i x = load_k16<BITS>(rr, o + (u32)(kc * KB + 16 * g * BITS / 8));
-                    if constexpr (BITS == 8) {
-                        if constexpr (!SGN) {
+            for (int i = 0; i < HOME; ++i) {
+                const int v = 4 * g + i;
 #pragma unroll
-                            for (int q = 0; q < 4; ++q) x[q] = (int)((u32)x[q] ^ 0x80808080u);
-                        }
-                    } else {
-                        x = expand_nibbles(x, false, SGN ? 0x88888888u : 0u);
-                    }
-                    bop[t][kc][0] = x;
-                }
+                for (int p = 0; p < P; ++p) rs_home[i] += D::weight(p) * lane_read(s, (p * V + v) & 15);
             }
-#pragma unroll
-            for (int t = 0; t < TPW; ++t) {
-                v4i acc[BD];
-#pragma unroll
-                for (int q = 0; q < BD; ++q) {
-                    acc[q] = v4i{0, 0, 0, 0};
-#pragma unroll
-                    for (int kc = 0; kc < NKC; ++kc) acc[q] = mfma_i8(aop[kc], bop[t][kc][q], acc[q]);
+        }
+
+        for (int c0 = start; c0 < wend; c0 += kSddmmStage) {
+            const int cn = min(wend - c0, kSddmmStage);
+            __syncthreads();
+            for (int i = threadIdx.x; i < ((cn + 15) & ~15); i += 64 * W) {
+                u32 o = kSddmmOob;
+                if (c0 + i < end) {
+                    const int idx = args.col_indices[c0 + i];
+                    if (idx >= 0) o = (u32)idx * row_bytes;
                 }
-                // rhs digit sum of nonzero r over all of K (16-bit only)
-                u32 corr = 0;
-                if constexpr (W16) {
-                    int sb = bsum[t];
-                    sb += lane_read(sb, lane ^ 16);
-                    sb += lane_read(sb, lane ^ 32);
-                    corr = (u32)D::OA * (u32)sb + (u32)args.k * (u32)D::OA * (u32)OB;
-                }
-                // recombine digit planes into the home lanes (rows v = 4g + i),
-                // mod 2^32 in unsigned arithmetic
-                int res[HOME];
+                s_off[i] = o;
+            }
+            __syncthreads();
+            const int ctiles = (cn + 15) >> 4;
+            // wave w handles tile groups w, w + W, ... of TPW tiles each
+            for (int tg = wave * TPW; tg < ctiles; tg += W * TPW) {
+                constexpr int BD = W16 ? 2 : 1;   // rhs digits (16-bit: low, high)
+                v4i bop[TPW][NKC][BD];
+                int bsum[TPW];                    // sum of rhs' over this lane's k (16-bit)
 #pragma unroll
-                for (int i = 0; i < HOME; ++i) {
-                    u32 vsum = (u32)OB * (u32)rs_home[i] + corr;
+                for (int t = 0; t < TPW; ++t) {
+                    const int tile = min(tg + t, ctiles - 1);
+                    const u32 add = (tg + t < ctiles) ? 0u : kSddmmOob;
+                    const u32 o = s_off[tile * 16 + r] + add;
+                    bsum[t] = 0;
 #pragma unroll
-                    for (int p = 0; p < P; ++p) {
-                        int dg, di;
-                        if constexpr (V == 8) { dg = 2 * (p % 2); di = 0; }
-                        else if constexpr (V == 4) { dg = p % 4; di = 0; }
-                        else { dg = (p % 8) / 2; di = 2 * (p % 2); }
-                        u32 x = (u32)acc[0][i + di];
-                        if constexpr (W16) x += (u32)acc[1][i + di] << 8;
-                        if (dg != 0) x = (u32)lane_down((int)x, 16 * dg);
-                        vsum += (u32)D::weight(p) * x;
+                    for (int kc = 0; kc < NKC; ++kc) {
+                        // chunks past the pass read out of range (B < 1 GiB, so
+                        // o + kSddmmOob stays below 2^32 and outside the buffer)
+                        const u32 ko = o + (kc < kn ? (u32)((kc0 + kc) * KB) : kSddmmOob);
+                        if constexpr (W16) {
+                            v4i hi, lo;
+                            load_k16_w16(rr, ko + (u32)(32 * g), XHI, XLO, hi, lo);
+                            if (kc >= kn) { hi = v4i{0, 0, 0, 0}; lo = v4i{0, 0, 0, 0}; }
+                            bop[t][kc][0] = lo;
+                            bop[t][kc][BD - 1] = hi;
+                            bsum[t] += 256 * sum_bytes(hi) + sum_bytes(lo);
+                            continue;
+                        }
+                        v4i x = load_k16<BITS>(rr, ko + (u32)(16 * g * BITS / 8));
+                        if constexpr (BITS == 8) {
+                            if constexpr (!SGN) {
+#pragma unroll
+                                for (int q = 0; q < 4; ++q) x[q] = (int)((u32)x[q] ^ 0x80808080u);
+                            }
+                        } else {
+                            x = expand_nibbles(x, false, SGN ? 0x88888888u : 0u);
+                        }
+                        bop[t][kc][0] = x;
                     }
-                    res[i] = (int)vsum;
                 }
-                const int j = c0 + (tg + t) * 16 + r;
-                if (g < (V + 3) / 4 && tg + t < ctiles && j < wend) {
+#pragma unroll
+                for (int t = 0; t < TPW; ++t) {
+                    v4i acc[BD];
+#pragma unroll
+                    for (int q = 0; q < BD; ++q) {
+                        acc[q] = v4i{0, 0, 0, 0};
+#pragma unroll
+                        for (int kc = 0; kc < NKC; ++kc) acc[q] = mfma_i8(aop[kc], bop[t][kc][q], acc[q]);
+                    }
+                    // rhs digit sum of nonzero r over the pass's k (16-bit only)
+                    u32 corr = 0;
+                    if constexpr (W16) {
+                        int sb = bsum[t];
+                        sb += lane_read(sb, lane ^ 16);
+                        sb += lane_read(sb, lane ^ 32);
+                        corr = (u32)D::OA * (u32)sb + (u32)(64 * kn) * (u32)D::OA * (u32)OB;
+                    }
+                    // recombine digit planes into the home lanes (rows v = 4g + i),
+                    // mod 2^32 in unsigned arithmetic
+                    int res[HOME];
 #pragma unroll
                     for (int i = 0; i < HOME; ++i) {
-                        const int v = 4 * g + i;
-                        const int val = j < end ? res[i] : 0;
-                        const size_t o = OUT_REF ? (size_t)(j >> 3) * 8 * V + 8 * v + (j & 7) : (size_t)j * V + v;
-                        args.out[o] = val;
+                        u32 vsum = (u32)OB * (u32)rs_home[i] + corr;
+#pragma unroll
+                        for (int p = 0; p < P; ++p) {
+                            int dg, di;
+                            if constexpr (V == 8) { dg = 2 * (p % 2); di = 0; }
+                            else if constexpr (V == 4) { dg = p % 4; di = 0; }
+                            else { dg = (p % 8) / 2; di = 2 * (p % 2); }
+                            u32 x = (u32)acc[0][i + di];
+                            if constexpr (W16) x += (u32)acc[1][i + di] << 8;
+                            if (dg != 0) x = (u32)lane_down((int)x, 16 * dg);
+                            vsum += (u32)D::weight(p) * x;
+                        }
+                        res[i] = (int)vsum;
+                    }
+                    const int j = c0 + (tg + t) * 16 + r;
+                    if (g < (V + 3) / 4 && tg + t < ctiles && j < wend) {
+#pragma unroll
+                        for (int i = 0; i < HOME; ++i) {
+                            const int v = 4 * g + i;
+                            const int val = j < end ? res[i] : 0;
+                            const size_t o =
+                                OUT_REF ? (size_t)(j >> 3) * 8 * V + 8 * v + (j & 7) : (size_t)j * V + v;
+                            // later passes add to what this lane stored before
+                            args.out[o] = ps == 0 ? val : (int)((u32)args.out[o] + (u32)val);
+                        }
                     }
                 }
             }

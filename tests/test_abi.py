@@ -116,7 +116,7 @@ def test_sddmm_validation(mclib):
     for fn in (mclib.mc_sddmm_l8r8, mclib.mc_sddmm_l4r4, mclib.mc_sddmm_l16r16):
         assert fn(4, 100, 64, None, p, p, p, p, p, 8, 0, None) == _lib.MC_EINVAL_SHAPE   # k % 64
         assert fn(4, 64, 64, None, p, p, p, p, p, 3, 0, None) == _lib.MC_EINVAL_V
-        assert fn(4, 512, 64, None, p, p, p, p, p, 8, 0, None) == _lib.MC_EUNSUPPORTED   # k not built
+        assert fn(4, 1 << 12, 1 << 20, None, p, p, p, p, p, 8, 0, None) == _lib.MC_EINVAL_SHAPE  # rhs over 1 GiB
         assert fn(4, 64, 64, None, None, p, p, p, p, 8, 0, None) == _lib.MC_EINVAL_ARG
         assert fn(4, 64, 64, None, p, p, p, p, p, 8, 5, None) == _lib.MC_EINVAL_ARG
         assert fn(0, 64, 64, None, p, p, p, p, p, 8, 0, None) == _lib.MC_OK

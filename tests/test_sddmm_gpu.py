@@ -48,7 +48,8 @@ def expected(orc, dm, csr, lhs, rhs, K, V):
     return exp
 
 
-@pytest.mark.parametrize("K", [64, 128, 256])
+# 192: a pass with an empty chunk; 640: a full pass then a partial one
+@pytest.mark.parametrize("K", [64, 128, 192, 256, 512, 640, 1024])
 @pytest.mark.parametrize("signed", [True, False])
 @pytest.mark.parametrize("V", [2, 4, 8])
 @pytest.mark.parametrize("bits", [8, 4, 16])
@@ -114,4 +115,14 @@ def test_sddmm_16b_extreme_values(orc, signed):
     lhs = np.where(lhs % 2 == 0, lo, hi).astype(np.int32)
     rhs = np.where(rhs % 3 == 0, lo, hi).astype(np.int32)
     dm, got = run(16, V, signed, csr, lhs, rhs, K)
+    assert np.array_equal(got, expected(orc, dm, csr, lhs, rhs, K, V))
+
+
+@pytest.mark.parametrize("bits", [8, 16])
+def test_sddmm_published_k512_shape_exact(orc, bits):
+    # the reference's published SDDMM configuration: 256 vector rows of V=8,
+    # N = 2304, K = 512, 90% sparse (one pass of eight 64-deep chunks)
+    V, K = 8, 512
+    csr, lhs, rhs = make(bits, V, False, M=2048, N=2304, K=K, sparsity=0.9, seed=41)
+    dm, got = run(bits, V, False, csr, lhs, rhs, K)
     assert np.array_equal(got, expected(orc, dm, csr, lhs, rhs, K, V))
