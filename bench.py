@@ -293,6 +293,70 @@ def sddmm_published_bench(steps, warmup, seed=43):
             "a100_source": "plot/examples/sddmm_abl_study.txt", "cases": cases}
 
 
+# The reference's 4-layer encoder on A100 (embed 256, 4 heads, MLP 1024, seq
+# 4096, sparsity 0.9, V = 8, fp16 model), mean of 8 runs:
+# (batch, lhs, rhs) -> (ms, plot/examples/magicube_n2n.txt lines); dense from pytorch_n2n.txt.
+PUBLISHED_ENCODER = {(2, 8, 8): (3.731, "19-26"), (2, 8, 4): (3.371, "10-17"),
+                     (8, 8, 8): (13.261, "55-62"), (8, 8, 4): (12.459, "46-53")}
+PUBLISHED_ENCODER_DENSE = {2: (5.947, "1-8"), 8: (24.541, "10-17")}
+
+
+def encoder_bench(reps=10, warmup=3, seed=0):
+    """The reference's end-to-end encoder benchmark (end_to_end.py): four
+    sparse blocks at seq 4096, embed 256, 4 heads, MLP 1024, sparsity 0.9,
+    V = 8, fp16 model, inference forward over random tokens; per batch size
+    and (lhs, rhs) precision the eager forward (HIP events around ``reps``
+    forwards) and the same forward replayed from a HIP graph; the dense
+    encoder (torch multi-head attention) beside it."""
+    import torch
+    from magicube_amd import spattention
+    cases = []
+    stream = torch.cuda.Stream()
+
+    def timed(fn):
+        with torch.cuda.stream(stream):
+            for _ in range(warmup):
+                fn()
+            t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            stream.synchronize()
+            t0.record(stream)
+            for _ in range(reps):
+                fn()
+            t1.record(stream)
+            t1.synchronize()
+        return t0.elapsed_time(t1) / reps
+
+    torch.manual_seed(seed)
+    with torch.no_grad():
+        for bs in (2, 8):
+            x = torch.randint(0, 257, (bs, 4096), dtype=torch.int32, device="cuda")
+            for (lp, rp) in ((8, 8), (8, 4)):
+                model = spattention.SparseTransformerEncoder(257, 256, 4, 0.9, 4096, 1024, 1024, vec_length=8,
+                                                             lhs_pre=lp, rhs_pre=rp, seed=seed).cuda().half().eval()
+                eager = timed(lambda: model(x))
+                with torch.cuda.stream(stream):
+                    g = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(g, stream=stream):
+                        out = model(x)
+                graph = timed(g.replay)
+                ok = bool(torch.isfinite(out.float()).all())
+                pub = PUBLISHED_ENCODER[(bs, lp, rp)]
+                cases.append({"batch": bs, "lhs_pre": lp, "rhs_pre": rp, "eager_ms": eager, "graph_ms": graph,
+                              "a100_ms": pub[0], "speedup_vs_a100": pub[0] / eager, "finite": ok})
+                del model, g, out
+            dense = spattention.TransformerEncoder(257, 256, 4, 4096, 1024, 1024).cuda().half().eval()
+            dms = timed(lambda: dense(x))
+            dpub = PUBLISHED_ENCODER_DENSE[bs]
+            cases.append({"batch": bs, "dense": True, "eager_ms": dms, "a100_ms": dpub[0],
+                          "speedup_vs_a100": dpub[0] / dms})
+            del dense
+            torch.cuda.empty_cache()
+    return {"workload": "4-layer encoder, seq 4096, embed 256, 4 heads, MLP 1024, sparsity 0.9, V=8, fp16, "
+                        "random tokens and weights, inference forward",
+            "a100_source": "plot/examples/magicube_n2n.txt, plot/examples/pytorch_n2n.txt",
+            "note": "rhs_pre 4 keeps V int8 (the reference quantizes V to 4 bits)", "cases": cases}
+
+
 def dense_attention_fp64_torch(q, k, v, mask, V):
     """softmax(Q K^T / 8) V in float64 on the GPU over the vector-sparse mask
     (rows of a vector row with no column give zeros): the FP64 reference the
@@ -454,6 +518,7 @@ def main():
     ap.add_argument("--no-attention", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-sddmm", action="store_true")
+    ap.add_argument("--no-encoder", action="store_true", help="skip the 4-layer encoder benchmark")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the every-pair sweep and the published-shape SDDMM")
     ap.add_argument("--config", type=int, default=None, help="explicit SpMM tile config index")
@@ -547,6 +612,9 @@ def main():
 
     if not args.no_serving:
         result["serving"] = serving_bench()
+
+    if not args.no_encoder:
+        result["encoder"] = encoder_bench()
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(csr, a_packed, rhs_np, M, K, N, V)

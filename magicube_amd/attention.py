@@ -170,13 +170,17 @@ def working_set_bytes(L: int, qk_bits: int) -> int:
 
 
 def make_resident(m: AttentionMatrix, scales: tuple, qk_bits: int, V: int, p_scale: float, device,
-                  debug: bool = False) -> _Dev:
+                  debug: bool = False, out: torch.Tensor | None = None) -> _Dev:
     """Uploads and aligns the mask when it is a host mask, and allocates the
-    matrix's quantized operands and output."""
+    matrix's quantized operands and output (or writes into ``out``, a
+    contiguous fp32 [L, 64] device tensor)."""
     mask = m.mask if isinstance(m.mask, ResidentMask) else ResidentMask.from_csr(m.mask, device)
     if m.q.shape[1] != HEAD_DIM or mask.V != V or mask.m_vec * V != m.L:
         raise ValueError("matrix shapes do not match the batch")
     qk_row = HEAD_DIM * qk_bits // 8
+    if out is not None and (tuple(out.shape) != (m.L, HEAD_DIM) or out.dtype != torch.float32
+                            or not out.is_contiguous()):
+        raise ValueError("out must be a contiguous float32 [L, 64] tensor")
     sq, sk, sv = (float(x) for x in scales)
     alpha = float(np.float32(1.0) / (np.float32(sq) * np.float32(sk) * np.float32(math.sqrt(HEAD_DIM))))
     o_scale = float(np.float32(1.0) / (np.float32(sv) * np.float32(p_scale)))
@@ -186,7 +190,7 @@ def make_resident(m: AttentionMatrix, scales: tuple, qk_bits: int, V: int, p_sca
              qq=torch.empty(max(m.L * qk_row, 16), dtype=torch.uint8, device=device),
              kq=torch.empty(max(m.L * qk_row, 16), dtype=torch.uint8, device=device),
              vq=torch.empty(max(m.L * HEAD_DIM, 16), dtype=torch.uint8, device=device),
-             o=torch.empty((m.L, HEAD_DIM), dtype=torch.float32, device=device),
+             o=out if out is not None else torch.empty((m.L, HEAD_DIM), dtype=torch.float32, device=device),
              scales=(sq, sk, sv), alpha=alpha, o_scale=o_scale, qk_bits=qk_bits)
     if debug:
         an = mask.aligned_num
@@ -263,7 +267,8 @@ class AttentionBatch:
 
     def __init__(self, mats: list[AttentionMatrix], qk_bits: int = 8, V: int = 8, p_mode: int = P_ROW_SCALED,
                  p_scale: float = 127.0, scales: list[tuple] | None = None, debug: bool = False,
-                 rounding: int | None = None, device="cuda", resident: list[_Dev] | None = None):
+                 rounding: int | None = None, device="cuda", resident: list[_Dev] | None = None,
+                 outputs: list[torch.Tensor] | None = None):
         if qk_bits not in (8, 4):
             raise ValueError("qk_bits must be 8 or 4")
         self.qk_bits, self.V, self.p_mode, self.p_scale = qk_bits, V, p_mode, float(p_scale)
@@ -279,8 +284,11 @@ class AttentionBatch:
             if scales is None:
                 flat = tensor_scales([t for m in mats for t in (m.q, m.k, m.v)])
                 scales = [tuple(flat[3 * i:3 * i + 3]) for i in range(len(mats))]
-            self.mats = [make_resident(m, sc, qk_bits, V, self.p_scale, self.device, debug)
-                         for m, sc in zip(mats, scales)]
+            outs = outputs if outputs is not None else [None] * len(mats)
+            if len(outs) != len(mats):
+                raise ValueError("one output tensor per matrix")
+            self.mats = [make_resident(m, sc, qk_bits, V, self.p_scale, self.device, debug, o)
+                         for m, sc, o in zip(mats, scales, outs)]
         self._assemble()
         self.graph = None
 

@@ -1,0 +1,266 @@
+"""Sparse multi-head attention and the 4-layer sparse encoder of the
+reference's end-to-end evaluation, on the fused MI355X attention kernel.
+
+``SpMultiheadAttention`` mirrors ``spMultiheadAttention``
+(end2end_eval/sparse_transformer_magicube/spattention.py:233-250): a
+``torch.nn.MultiheadAttention`` whose forward also takes the vector-sparse
+mask (``row_indices``, ``row_offsets``, ``column_indices`` as
+``static_random_mask_aligned`` builds them, verify/static_mask.py:29-76), the
+vector length and the operand precisions.  Per (sample, head) it computes,
+as the reference does (spattention.py:51-53, 182-221):
+
+    Q, K, V = in-projection of the input, quantized with the fixed scale 36
+              (truncation, clamped)
+    P       = trunc(min(softmax(Q K^T / (36 * 36) / sqrt(64)) * 32, 127))
+              over the mask's nonzeros
+    O       = P V / (36 * 32), then the output projection.
+
+All (sample, head) matrices run in one persistent launch (``mc_attn_forward``)
+over the shared mask.  The module keeps its staging buffers, quantized
+operands, outputs and launch descriptors per (shape, mask), so later calls
+only launch kernels and can be captured in a HIP graph.
+
+Precisions: ``lhs_pre`` 8 (int8 P) with ``rhs_pre`` 8 (int8 Q, K, V) or 4
+(int4 Q, K).  V stays int8 in both: the reference quantizes V to ``rhs_pre``
+bits too (spattention.py:191), so with ``rhs_pre`` 4 this layer's V is more
+precise than the reference's.  ``need_weights=True`` and
+``key_padding_mask`` are rejected; the reference's encoder uses neither
+(end_to_end.py:70-73).
+
+``SparseTransformerEncoder`` / ``TransformerEncoder`` restate the encoder
+benchmark models (end_to_end.py:47-183): embedding, sinusoidal positions,
+four blocks of LayerNorm -> attention -> residual -> LayerNorm -> Linear ->
+Linear -> residual, then LayerNorm and a Linear head.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+from torch import nn
+
+from . import formats
+from .attention import HEAD_DIM, P_REFERENCE, ROUND_TRUNC, AttentionBatch, AttentionMatrix, ResidentMask
+
+SCALE_QKV = 36.0    # spattention.py:51
+SCALE_SFMX = 32.0   # spattention.py:53
+
+
+def static_random_mask_aligned(m: int, n: int, sparsity: float, mma_k_dim: int, seed: int | None = None,
+                               device="cuda"):
+    """The reference's attention mask builder (verify/static_mask.py:29-76):
+    an element-random m x n pattern of density 1 - sparsity (each vector row
+    of the attention matrix is one row here), row pairs padded to
+    ``mma_k_dim`` slots, -1-padded column indices, the same indices shuffled
+    in groups of 8, and the rows ordered by ascending length.  Returns
+    (column_indices, column_indices_shuffle, row_offsets, row_indices,
+    aligned_num_item) with the tensors on ``device``."""
+    rng = np.random.default_rng(seed)
+    k = int(round((1.0 - sparsity) * m * n))
+    flat = np.sort(rng.choice(m * n, size=k, replace=False))
+    rows, cols = flat // n, (flat % n).astype(np.int32)
+    counts = np.bincount(rows, minlength=m)
+    ro = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    pairs, an = formats.align_rows(ro, mma_k_dim)
+    aligned = formats.align_cols(ro, cols, pairs, an, shuffle=False)
+    shuffled = formats.align_cols(ro, cols, pairs, an, shuffle=True)
+    order = np.argsort(counts, kind="stable").astype(np.int32)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(device)  # noqa: E731
+    return t(aligned), t(shuffled), t(pairs), t(order), int(an)
+
+
+def mask_from_reference(row_indices: torch.Tensor, row_offsets: torch.Tensor, column_indices: torch.Tensor,
+                        seq_len: int, vec_length: int) -> ResidentMask:
+    """A :class:`ResidentMask` over the reference's device mask tensors
+    (aligned (start, end) pairs, -1-padded unshuffled column indices).  The
+    tensors are used in place; row starts must be multiples of 16 slots
+    (alignment 16 or 32, as the reference pads for 8- and 4-bit operands)."""
+    pairs = row_offsets.detach().to("cpu", torch.int32).numpy()
+    m_vec = pairs.size // 2
+    if m_vec * vec_length != seq_len:
+        raise ValueError(f"mask has {m_vec} vector rows of {vec_length}, sequence length is {seq_len}")
+    starts, ends = pairs[0::2], pairs[1::2]
+    if (starts % 16).any() or (ends < starts).any() or (ends.max(initial=0) > column_indices.numel()):
+        raise ValueError("row_offsets must hold (start, end) pairs with starts aligned to 16 slots")
+    ro = row_offsets.to(torch.int32).contiguous()
+    return ResidentMask(L=seq_len, V=vec_length, m_vec=m_vec, nnz_vec=int((ends - starts).sum()),
+                        aligned_num=int(column_indices.numel()), row_offsets=ro,
+                        col_indices=column_indices.to(torch.int32).contiguous(),
+                        row_indices=row_indices.to(torch.int32).contiguous(),
+                        row_nnz=(ro[1::2] - ro[0::2]).contiguous(), pairs=pairs)
+
+
+class _Plan:
+    """Per (shape, mask, precision) state of SpMultiheadAttention."""
+
+    def __init__(self, L: int, n_mat: int, mask: ResidentMask, rhs_pre: int, device):
+        self.stage = torch.empty((3, n_mat, L, HEAD_DIM), dtype=torch.float16, device=device)
+        self.out = torch.empty((n_mat, L, HEAD_DIM), dtype=torch.float32, device=device)
+        mats = [AttentionMatrix(self.stage[0, i], self.stage[1, i], self.stage[2, i], mask) for i in range(n_mat)]
+        self.batch = AttentionBatch(mats, qk_bits=rhs_pre, V=mask.V, p_mode=P_REFERENCE, p_scale=SCALE_SFMX,
+                                    scales=[(SCALE_QKV, SCALE_QKV, SCALE_QKV)] * n_mat, rounding=ROUND_TRUNC,
+                                    device=device, outputs=[self.out[i] for i in range(n_mat)])
+
+
+class SpMultiheadAttention(nn.MultiheadAttention):
+    """Drop-in for the reference's ``spMultiheadAttention`` (sequence-first
+    [L, batch, embed] inputs, head dimension 64)."""
+
+    def __init__(self, embed_dim, num_heads, dropout=0.0, bias=True, add_bias_kv=False, add_zero_attn=False,
+                 kdim=None, vdim=None):
+        super().__init__(embed_dim, num_heads, dropout, bias, add_bias_kv, add_zero_attn, kdim, vdim)
+        if embed_dim // num_heads != HEAD_DIM:
+            raise ValueError(f"head dimension must be {HEAD_DIM}")
+        if add_bias_kv or add_zero_attn:
+            raise ValueError("bias_kv and zero attention are not supported")
+        self._plans: dict = {}
+
+    def forward(self, query, key, value, key_padding_mask=None, need_weights=True, row_indices=None,
+                row_offsets=None, column_indices=None, vec_length=2, lhs_pre=8, rhs_pre=8, mask=None):
+        if need_weights:
+            raise NotImplementedError("attention weights are not materialized: call with need_weights=False")
+        if key_padding_mask is not None:
+            raise NotImplementedError("key_padding_mask is not supported")
+        if lhs_pre != 8 or rhs_pre not in (8, 4):
+            raise NotImplementedError(f"precisions lhs_pre={lhs_pre}, rhs_pre={rhs_pre}: supported are "
+                                      "lhs_pre 8 with rhs_pre 8 or 4")
+        L, bsz, E = query.shape
+        H = self.num_heads
+        if key.shape != query.shape or value.shape != query.shape:
+            raise ValueError("query, key and value must have the same shape")
+        if mask is None:
+            if row_offsets is None or column_indices is None:
+                raise ValueError("a mask (row_offsets, column_indices) is required")
+            key_ = (L, bsz, rhs_pre, vec_length, row_offsets.data_ptr(), column_indices.data_ptr())
+        else:
+            key_ = (L, bsz, rhs_pre, mask.V, id(mask))
+        plan = self._plans.get(key_)
+        if plan is None:
+            if mask is None:
+                ri = row_indices if row_indices is not None else torch.zeros(1, dtype=torch.int32,
+                                                                               device=query.device)
+                mask = mask_from_reference(ri, row_offsets, column_indices, L, vec_length)
+            plan = _Plan(L, bsz * H, mask, rhs_pre, query.device)
+            self._plans[key_] = plan
+
+        # in-projection; stage Q, K, V as [3][sample * head][L][64] fp16
+        if query is key and key is value:
+            qkv = F.linear(query, self.in_proj_weight, self.in_proj_bias)
+        else:
+            w, b = self.in_proj_weight, self.in_proj_bias
+            qkv = torch.cat([F.linear(x, w[i * E:(i + 1) * E], None if b is None else b[i * E:(i + 1) * E])
+                             for i, x in enumerate((query, key, value))], dim=-1)
+        plan.stage.view(3, bsz, H, L, HEAD_DIM).copy_(qkv.view(L, bsz, 3, H, HEAD_DIM).permute(2, 1, 3, 0, 4))
+        plan.batch.forward()
+        o = torch.empty((L, bsz, E), dtype=query.dtype, device=query.device)
+        o.view(L, bsz, H, HEAD_DIM).copy_(plan.out.view(bsz, H, L, HEAD_DIM).permute(2, 0, 1, 3))
+        return F.linear(o, self.out_proj.weight, self.out_proj.bias), None
+
+
+class PositionalEncoding(nn.Module):
+    """Sinusoidal positions added to [L, batch, embed] (end_to_end.py:115-130)."""
+
+    def __init__(self, d_model: int, max_len: int = 10000):
+        super().__init__()
+        position = torch.arange(0, max_len, dtype=torch.float).unsqueeze(1)
+        div_term = torch.exp(torch.arange(0, d_model, 2).float() * (-math.log(10000.0) / d_model))
+        pe = torch.zeros(max_len, d_model)
+        pe[:, 0::2] = torch.sin(position * div_term)
+        pe[:, 1::2] = torch.cos(position * div_term)
+        self.register_buffer("pe", pe.unsqueeze(1))
+
+    def forward(self, x):
+        return x + self.pe[:x.size(0)]
+
+
+class SparseTransformerBlock(nn.Module):
+    """end_to_end.py:47-89: LayerNorm -> sparse attention -> residual ->
+    LayerNorm -> Linear -> Linear -> residual, with its own random mask."""
+
+    def __init__(self, embed_dim, mlp_dim, num_heads, sparsity, seq_len, vec_length=8, lhs_pre=8, rhs_pre=8,
+                 seed=None, device="cuda"):
+        super().__init__()
+        self.vec_length, self.lhs_pre, self.rhs_pre = vec_length, lhs_pre, rhs_pre
+        self.self_attention = SpMultiheadAttention(embed_dim=embed_dim, num_heads=num_heads, dropout=0)
+        mma_k_dim = 32 if rhs_pre == 4 else 16
+        (self.column_indices, self.column_indices_shuffle, self.row_offsets, self.row_indices,
+         self.aligned_num_item) = static_random_mask_aligned(seq_len // vec_length, seq_len, sparsity, mma_k_dim,
+                                                             seed=seed, device=device)
+        self.layer_norm1 = nn.LayerNorm(normalized_shape=embed_dim)
+        self.layer_norm2 = nn.LayerNorm(normalized_shape=embed_dim)
+        self.linear1 = nn.Linear(embed_dim, mlp_dim)
+        self.linear2 = nn.Linear(mlp_dim, embed_dim)
+
+    def forward(self, x):
+        out = self.layer_norm1(x)
+        out = self.self_attention(out, out, out, need_weights=False, row_indices=self.row_indices,
+                                  row_offsets=self.row_offsets, column_indices=self.column_indices,
+                                  vec_length=self.vec_length, lhs_pre=self.lhs_pre, rhs_pre=self.rhs_pre)
+        out = out[0] + x
+        out = self.layer_norm2(out)
+        out = self.linear1(out)
+        out = self.linear2(out)
+        return out + x
+
+
+class TransformerBlock(nn.Module):
+    """The dense block of the same benchmark (end_to_end.py:91-113), with
+    torch's multi-head attention."""
+
+    def __init__(self, embed_dim, mlp_dim, num_heads):
+        super().__init__()
+        self.self_attention = nn.MultiheadAttention(embed_dim=embed_dim, num_heads=num_heads, dropout=0)
+        self.layer_norm1 = nn.LayerNorm(normalized_shape=embed_dim)
+        self.layer_norm2 = nn.LayerNorm(normalized_shape=embed_dim)
+        self.linear1 = nn.Linear(embed_dim, mlp_dim)
+        self.linear2 = nn.Linear(mlp_dim, embed_dim)
+
+    def forward(self, x):
+        out = self.layer_norm1(x)
+        out = self.self_attention(out, out, out, need_weights=False)
+        out = out[0] + x
+        out = self.layer_norm2(out)
+        out = self.linear1(out)
+        out = self.linear2(out)
+        return out + x
+
+
+class _Encoder(nn.Module):
+    def __init__(self, vocab_size, embed_dim, num_class, blocks):
+        super().__init__()
+        self.encoder = nn.Embedding(num_embeddings=vocab_size, embedding_dim=embed_dim)
+        self.pos_encoder = PositionalEncoding(embed_dim)
+        self.blocks = nn.ModuleList(blocks)
+        self.embed_dim = embed_dim
+        self.layer_norm = nn.LayerNorm(normalized_shape=embed_dim)
+        self.linear = nn.Linear(embed_dim, num_class)
+
+    def forward(self, x):
+        out = self.encoder(x) * math.sqrt(self.embed_dim)
+        out = torch.transpose(out, 0, 1)
+        out = self.pos_encoder(out)
+        for blk in self.blocks:
+            out = blk(out)
+        out = self.layer_norm(out)
+        return self.linear(out)
+
+
+class SparseTransformerEncoder(_Encoder):
+    """end_to_end.py:133-157 (four sparse blocks)."""
+
+    def __init__(self, vocab_size, embed_dim, num_heads, sparsity, seq_len, mlp_dim, num_class, vec_length=8,
+                 lhs_pre=8, rhs_pre=8, seed=0, device="cuda", layers=4):
+        blocks = [SparseTransformerBlock(embed_dim, mlp_dim, num_heads, sparsity, seq_len, vec_length, lhs_pre,
+                                         rhs_pre, seed=None if seed is None else seed + i, device=device)
+                  for i in range(layers)]
+        super().__init__(vocab_size, embed_dim, num_class, blocks)
+
+
+class TransformerEncoder(_Encoder):
+    """end_to_end.py:160-183 (four dense blocks)."""
+
+    def __init__(self, vocab_size, embed_dim, num_heads, seq_len, mlp_dim, num_class, layers=4):
+        super().__init__(vocab_size, embed_dim, num_class,
+                         [TransformerBlock(embed_dim, mlp_dim, num_heads) for _ in range(layers)])

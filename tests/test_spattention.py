@@ -1,0 +1,55 @@
+"""Host-side parts of the sparse attention module (CPU): the reference's
+mask builder restated, the reference-mask adapter's checks and the module's
+argument checks."""
+import numpy as np
+import pytest
+import torch
+
+from magicube_amd import spattention
+
+
+@pytest.mark.parametrize("align", [16, 32])
+def test_static_random_mask_aligned_layout(align):
+    m, n, s = 64, 512, 0.9
+    cols, shuf, pairs, order, an = spattention.static_random_mask_aligned(m, n, s, align, seed=3, device="cpu")
+    cols, shuf, pairs, order = (t.numpy() for t in (cols, shuf, pairs, order))
+    starts, ends = pairs[0::2], pairs[1::2]
+    counts = ends - starts
+    assert counts.sum() == round((1 - s) * m * n)                 # exact density, as scipy.sparse.random
+    assert (starts % align == 0).all() and an % align == 0 and cols.size == an
+    assert np.array_equal(starts[1:], starts[:-1] + (counts[:-1] + align - 1) // align * align)
+    for i in range(m):
+        row = cols[starts[i]:ends[i]]
+        assert (np.diff(row) > 0).all() and row.min(initial=0) >= 0 and row.max(initial=0) < n
+        assert (cols[ends[i]:starts[i] + (counts[i] + align - 1) // align * align] == -1).all()
+    g = shuf.reshape(-1, 8)
+    assert np.array_equal(g[:, :4], cols.reshape(-1, 8)[:, 0::2]) and np.array_equal(g[:, 4:], cols.reshape(-1, 8)[:, 1::2])
+    assert (np.diff(counts[order]) >= 0).all()                    # ascending length, as np.argsort
+
+
+def test_mask_from_reference_checks():
+    cols, _, pairs, order, an = spattention.static_random_mask_aligned(32, 256, 0.8, 16, seed=1, device="cpu")
+    mk = spattention.mask_from_reference(order, pairs, cols, seq_len=256, vec_length=8)
+    assert mk.m_vec == 32 and mk.aligned_num == an and mk.nnz_vec == int((pairs[1::2] - pairs[0::2]).sum())
+    assert torch.equal(mk.row_nnz, pairs[1::2] - pairs[0::2])
+    with pytest.raises(ValueError):
+        spattention.mask_from_reference(order, pairs, cols, seq_len=512, vec_length=8)
+    bad = pairs.clone()
+    bad[2] += 1                                                   # a start off the 16-slot grid
+    with pytest.raises(ValueError):
+        spattention.mask_from_reference(order, bad, cols, seq_len=256, vec_length=8)
+
+
+def test_module_argument_checks():
+    mha = spattention.SpMultiheadAttention(embed_dim=128, num_heads=2)
+    x = torch.zeros(64, 1, 128)
+    cols, _, pairs, order, _ = spattention.static_random_mask_aligned(8, 64, 0.5, 16, seed=0, device="cpu")
+    kw = dict(row_indices=order, row_offsets=pairs, column_indices=cols, vec_length=8)
+    with pytest.raises(NotImplementedError):
+        mha(x, x, x, need_weights=True, **kw)
+    with pytest.raises(NotImplementedError):
+        mha(x, x, x, need_weights=False, lhs_pre=16, rhs_pre=8, **kw)
+    with pytest.raises(NotImplementedError):
+        mha(x, x, x, need_weights=False, key_padding_mask=torch.zeros(1, 64, dtype=torch.bool), **kw)
+    with pytest.raises(ValueError):
+        spattention.SpMultiheadAttention(embed_dim=96, num_heads=2)    # head dimension 48

@@ -1,0 +1,91 @@
+"""The sparse multi-head attention module and the encoder on the GPU: the
+module's attention equals the one-op-per-launch pipeline bitwise and the
+numpy restatement of the reference's quantized attention within one P step;
+the encoder is deterministic and replays in a HIP graph bitwise."""
+import numpy as np
+import pytest
+import torch
+
+import oracle.attention as oatt
+from magicube_amd import formats, spattention
+
+pytestmark = pytest.mark.gpu
+
+
+def _layer(L, bsz, heads, rhs_pre, seed):
+    torch.manual_seed(seed)
+    E = 64 * heads
+    mha = spattention.SpMultiheadAttention(embed_dim=E, num_heads=heads).cuda().half().eval()
+    align = 32 if rhs_pre == 4 else 16
+    cols, _, pairs, order, _ = spattention.static_random_mask_aligned(L // 8, L, 0.9, align, seed=seed)
+    x = (torch.randn(L, bsz, E, device="cuda") * 0.5).half()
+    return mha, x, dict(row_indices=order, row_offsets=pairs, column_indices=cols, vec_length=8, rhs_pre=rhs_pre)
+
+
+@pytest.mark.parametrize("rhs_pre", [8, 4])
+def test_module_equals_plain_pipeline_and_restatement(orc, rhs_pre):
+    L, bsz, heads = 512, 2, 2
+    mha, x, kw = _layer(L, bsz, heads, rhs_pre, seed=7)
+    print(f"spattention L={L} bsz={bsz} heads={heads} rhs_pre={rhs_pre}", flush=True)
+    with torch.no_grad():
+        out, _ = mha(x, x, x, need_weights=False, **kw)
+        torch.cuda.synchronize()
+        plan = next(iter(mha._plans.values()))
+        fused = plan.out.cpu().numpy().copy()
+        plain = [o.cpu().numpy() for o in plan.batch.forward_plain()]
+        torch.cuda.synchronize()
+    assert torch.isfinite(out).all()
+    for i in range(bsz * heads):
+        assert np.array_equal(fused[i].view(np.uint32), plain[i].view(np.uint32)), f"matrix {i}"
+    # the reference's formulas restated in numpy (static scales 36 / 32, truncation)
+    pairs = kw["row_offsets"].cpu().numpy()
+    cols = kw["column_indices"].cpu().numpy()
+    counts = pairs[1::2] - pairs[0::2]
+    ro = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    ci = np.concatenate([cols[pairs[2 * i]:pairs[2 * i + 1]] for i in range(L // 8)]).astype(np.int32)
+    csr = formats.VectorCSR(L // 8, L, 8, ro, ci)
+    stage = plan.stage.cpu().numpy()
+    for i in range(bsz * heads):
+        ref = oatt.quantized_attention_ref(stage[0, i], stage[1, i], stage[2, i], csr, (36.0, 36.0, 36.0),
+                                           qk_bits=rhs_pre, p_mode=0, p_scale=32.0, rounding=0)
+        # P agrees with the FP32 restatement to within one step (1/32 of a
+        # probability) at a few slots; each such slot moves O by at most |v| / 32
+        vmax = np.abs(ref["vq"]).max() / 36.0
+        diff = np.abs(fused[i] - ref["o"])
+        assert diff.max() <= 4 * vmax / 32.0 + 1e-6, f"matrix {i}"
+        assert np.linalg.norm(fused[i] - ref["o"]) <= 1e-2 * np.linalg.norm(ref["o"]), f"matrix {i}"
+
+
+def test_module_output_projection_and_layout():
+    L, bsz, heads = 256, 3, 2
+    mha, x, kw = _layer(L, bsz, heads, 8, seed=11)
+    with torch.no_grad():
+        out, _ = mha(x, x, x, need_weights=False, **kw)
+        plan = next(iter(mha._plans.values()))
+        # [sample * head][L][64] -> [L][sample][head * 64], then out_proj
+        o = plan.out.view(bsz, heads, L, 64).permute(2, 0, 1, 3).reshape(L, bsz, 64 * heads).half()
+        exp = torch.nn.functional.linear(o, mha.out_proj.weight, mha.out_proj.bias)
+    torch.cuda.synchronize()
+    assert torch.equal(out, exp)
+
+
+def test_encoder_deterministic_and_graph_replay():
+    torch.manual_seed(0)
+    model = spattention.SparseTransformerEncoder(vocab_size=257, embed_dim=256, num_heads=4, sparsity=0.9,
+                                                 seq_len=1024, mlp_dim=1024, num_class=2, vec_length=8,
+                                                 seed=5).cuda().half().eval()
+    x = torch.randint(0, 257, (2, 1024), dtype=torch.int32, device="cuda")
+    print("encoder seq 1024 bs 2", flush=True)
+    s = torch.cuda.Stream()
+    with torch.no_grad(), torch.cuda.stream(s):
+        a = model(x)
+        b = model(x)
+        s.synchronize()
+        assert torch.isfinite(a.float()).all() and torch.equal(a, b)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            c = model(x)
+        for _ in range(3):
+            g.replay()
+            s.synchronize()
+            assert torch.equal(c, a)
