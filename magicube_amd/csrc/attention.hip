@@ -169,6 +169,14 @@ __global__ __launch_bounds__(64 * WPB) void softmax_requant_kernel(int m_vec, co
     }
 }
 
+// Row bounds are read-only for the whole launch; read through the constant
+// address space at a wave-uniform address they become scalar loads (scalar
+// cache, lgkmcnt), which neither wait behind nor hold back vector gathers.
+__device__ __forceinline__ int const_load(const int *p) {
+    typedef const __attribute__((address_space(4))) int const_int;
+    return *(const const_int *)p;
+}
+
 // ---------------------------------------------------------------------------
 // Fused batched attention: persistent waves walk a work list of (matrix,
 // vector row) items.  Pass 1 gathers K rows, computes the int32 logits with
@@ -211,7 +219,8 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
         M.p_out = uniform(Mv.p_out);
         M.s_out = uniform(Mv.s_out);
         const int i_row = uniform(item.y);
-        const int start = uniform(M.row_offsets[2 * i_row]), end = uniform(M.row_offsets[2 * i_row + 1]);
+        const int start = uniform(const_load(M.row_offsets + 2 * i_row)),
+                  end = uniform(const_load(M.row_offsets + 2 * i_row + 1));
         const int aend = start + ((end - start + 15) & ~15);
         const int nch = (end - start + 63) >> 6;
         const int nch_w = (aend - start + 63) >> 6;

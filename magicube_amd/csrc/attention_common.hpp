@@ -102,17 +102,19 @@ __device__ __forceinline__ v4i p_to_a_layout(unsigned char *tile, const int (&pq
 #pragma unroll
             for (int i = 0; i < HOME; ++i) tile[(4 * g + i) * 64 + 16 * t + c] = (unsigned char)pq[t][i];
     }
-    // LDS operations of one wave complete in order; the fences keep the
-    // compiler from moving the read above the writes (and the next writes
-    // above this read).  Rows >= V are never written and only feed output
-    // rows that are discarded.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    // The tile is private to the wave and LDS operations of one wave complete
+    // in order, so wavefront-scope fences (no waits) are enough: they keep
+    // the compiler from moving the read above the writes (and the next writes
+    // above this read).  Workgroup scope would also wait for every gather in
+    // flight that lands in LDS.  Rows >= V are never written and only feed
+    // output rows that are discarded.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const v4i a = *reinterpret_cast<const v4i *>(tile + c * 64 + 16 * g);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     return a;
 }
 

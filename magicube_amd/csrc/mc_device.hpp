@@ -73,6 +73,14 @@ __device__ __forceinline__ v4i buf_load_b128(__amdgpu_buffer_rsrc_t r, u32 off) 
     return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
 }
 
+// ds_read_b64_tr_b8: within each 16-lane group, lane 8h + j
+// receives byte j of the 8-byte segments addressed by lanes h, h + 2, ...,
+// h + 14 (8 bytes: one column of an 8 x 8 byte block).
+typedef __attribute__((address_space(3))) v2i lds_v2i;
+__device__ __forceinline__ v2i lds_read_tr8(const unsigned char *base, u32 off) {
+    return __builtin_amdgcn_ds_read_tr8_b64_v2i32((lds_v2i *)(base + off));
+}
+
 // Wave-uniform copies of values every lane loaded from the same address.
 // Values in VGPRs cannot feed a buffer resource or a scalar branch without a
 // per-lane waterfall loop; one v_readfirstlane each makes them SGPR-resident.

@@ -25,12 +25,6 @@
 
 namespace mc {
 
-typedef __attribute__((address_space(3))) v2i lds_v2i;
-
-__device__ __forceinline__ v2i lds_read_tr8(const unsigned char *base, u32 off) {
-    return __builtin_amdgcn_ds_read_tr8_b64_v2i32((lds_v2i *)(base + off));
-}
-
 constexpr int kStatMaxRowsPerWave = 4;
 // LDS budget of the B slice (K + 1 rows of TB bytes), below gfx950's 160 KiB.
 constexpr int kStatLdsBytes = 160 * 1024 - 1024;
