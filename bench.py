@@ -47,6 +47,41 @@ PUBLISHED_SPMM_PRES = {
     (8, 4, 2): (4.63334, 1110), (8, 4, 4): (8.65352, 1121), (8, 4, 8): (20.4712, 1132),
     (4, 4, 2): (5.95672, 1310), (4, 4, 4): (11.6777, 1321), (4, 4, 8): (21.0431, 1332),
 }
+# The reference's SpMM ablation on A100 (DLMC rn50 random_pruning
+# bottleneck_2_block_group3_5_1: 256 vector rows, K = 2304, N = 512):
+# (stage, lhs, rhs, sparsity, V) -> (TOPS, plot/examples/spmm_abl_study.txt line).
+# Stages: basic kernel, + conflict-free shared memory, + prefetch, + shuffle
+# (the last for 4-bit B only).
+PUBLISHED_SPMM_ABLATION = {
+    ("basic", 16, 8, 0.7, 2): (3.7567, 12), ("basic", 16, 8, 0.7, 8): (15.0338, 24),
+    ("basic", 16, 8, 0.9, 2): (2.74522, 36), ("basic", 16, 8, 0.9, 8): (11.5881, 48),
+    ("basic", 8, 8, 0.7, 2): (3.85332, 62), ("basic", 8, 8, 0.7, 8): (15.3045, 74),
+    ("basic", 8, 8, 0.9, 2): (2.97658, 86), ("basic", 8, 8, 0.9, 8): (11.7619, 98),
+    ("basic", 8, 4, 0.7, 2): (4.55222, 112), ("basic", 8, 4, 0.7, 8): (18.5377, 123),
+    ("basic", 8, 4, 0.9, 2): (3.17841, 134), ("basic", 8, 4, 0.9, 8): (13.1074, 145),
+    ("basic", 4, 4, 0.7, 2): (4.93733, 159), ("basic", 4, 4, 0.7, 8): (19.0693, 171),
+    ("basic", 4, 4, 0.9, 2): (3.62971, 183), ("basic", 4, 4, 0.9, 8): (13.4169, 195),
+    ("conflict_free", 16, 8, 0.7, 2): (4.37232, 212), ("conflict_free", 16, 8, 0.7, 8): (17.4843, 224),
+    ("conflict_free", 16, 8, 0.9, 2): (3.03427, 236), ("conflict_free", 16, 8, 0.9, 8): (13.0248, 248),
+    ("conflict_free", 8, 8, 0.7, 2): (4.47763, 265), ("conflict_free", 8, 8, 0.7, 8): (17.8471, 277),
+    ("conflict_free", 8, 8, 0.9, 2): (3.41112, 289), ("conflict_free", 8, 8, 0.9, 8): (13.2815, 301),
+    ("conflict_free", 8, 4, 0.7, 2): (4.7151, 318), ("conflict_free", 8, 4, 0.7, 8): (18.9403, 329),
+    ("conflict_free", 8, 4, 0.9, 2): (3.25536, 340), ("conflict_free", 8, 4, 0.9, 8): (13.3273, 351),
+    ("conflict_free", 4, 4, 0.7, 2): (4.95606, 368), ("conflict_free", 4, 4, 0.7, 8): (19.4581, 380),
+    ("conflict_free", 4, 4, 0.9, 2): (3.65532, 392), ("conflict_free", 4, 4, 0.9, 8): (13.6976, 404),
+    ("prefetch", 16, 8, 0.7, 2): (5.53128, 421), ("prefetch", 16, 8, 0.7, 8): (21.4967, 433),
+    ("prefetch", 16, 8, 0.9, 2): (3.6021, 445), ("prefetch", 16, 8, 0.9, 8): (15.7921, 457),
+    ("prefetch", 8, 8, 0.7, 2): (5.86861, 474), ("prefetch", 8, 8, 0.7, 8): (22.9959, 486),
+    ("prefetch", 8, 8, 0.9, 2): (4.32687, 498), ("prefetch", 8, 8, 0.9, 8): (16.5091, 510),
+    ("prefetch", 8, 4, 0.7, 2): (6.23654, 527), ("prefetch", 8, 4, 0.7, 8): (24.4017, 538),
+    ("prefetch", 8, 4, 0.9, 2): (4.02061, 549), ("prefetch", 8, 4, 0.9, 8): (16.6622, 560),
+    ("prefetch", 4, 4, 0.7, 2): (6.65973, 577), ("prefetch", 4, 4, 0.7, 8): (25.4291, 589),
+    ("prefetch", 4, 4, 0.9, 2): (4.6343, 601), ("prefetch", 4, 4, 0.9, 8): (17.2339, 613),
+    ("shuffle", 8, 4, 0.7, 2): (8.49537, 630), ("shuffle", 8, 4, 0.7, 8): (34.7546, 642),
+    ("shuffle", 8, 4, 0.9, 2): (4.64603, 654), ("shuffle", 8, 4, 0.9, 8): (20.5391, 666),
+    ("shuffle", 4, 4, 0.7, 2): (9.52411, 684), ("shuffle", 4, 4, 0.7, 8): (36.9262, 697),
+    ("shuffle", 4, 4, 0.9, 2): (5.9758, 710), ("shuffle", 4, 4, 0.9, 8): (21.0911, 723),
+}
 # SDDMM on A100 at the same matrix, N = 2304, K = 512 (basic kernel):
 # (bits, V) -> (TOPS, plot/examples/sddmm_abl_study.txt line)
 PUBLISHED_SDDMM_K512 = {
@@ -242,6 +277,54 @@ def precision_bench(steps, warmup, seed=17):
                           "ratio_vs_a100": tops / pub[0] if pub else None, "verified": ok})
     return {"workload": "SpMM m_vec=256, K=2304, N=512, s=0.9 (Poisson vector-sparse), unsigned, every pair x V",
             "a100_source": "plot/examples/spmm_pres.txt", "cases": cases}
+
+
+def ablation_bench(steps, warmup, seed=23):
+    """SpMM ablation at the reference's ablation shape (256 vector rows,
+    K = 2304, N = 512, s in {0.7, 0.9}, V in {2, 8}, unsigned; pairs L16-R8,
+    L8-R8, L8-R4, L4-R4), beside its A100 stages.  The stages here are this
+    kernel's own design steps: gathers of one, two or three 64-slot k-blocks in
+    flight per wave (tile configs 13, 14, 7), the per-pair default tile, and
+    the default with vector rows in their stored order instead of sorted by
+    length."""
+    import torch
+    from magicube_amd import spmm
+    stream = torch.cuda.Stream()
+    K, N = 2304, 512
+    stages = (("one_kblock_in_flight", 13, "sorted"), ("two_kblocks_in_flight", 14, "sorted"),
+              ("three_kblocks_in_flight", 7, "sorted"), ("default", None, "sorted"),
+              ("default_rows_unsorted", None, "identity"))
+    cases = []
+    for (la, lb) in ((16, 8), (8, 8), (8, 4), (4, 4)):
+        for s in (0.7, 0.9):
+            for V in (2, 8):
+                M = 256 * V
+                csr, a, b, _, prep, rhs_np = make_workload(M, K, N, V, s, seed, la, lb, signed=False)
+                dev = spmm.DeviceSpmm.upload(prep)
+                rhs = torch.from_numpy(rhs_np).cuda()
+                out = torch.empty((M, N), dtype=torch.int32, device="cuda")
+                sorted_rows = dev.row_indices
+                identity = torch.arange(csr.m_vec, dtype=torch.int32, device="cuda")
+                row = {"pair": f"L{la}-R{lb}", "sparsity": s, "V": V, "nnz_vec": csr.nnz_vec, "tops": {},
+                       "verified": True}
+                for name, cfg, order in stages:
+                    dev.row_indices = sorted_rows if order == "sorted" else identity
+
+                    def run():
+                        dev.matmul(rhs, N, out=out, sign=spmm.UNSIGNED, config=cfg, stream=stream)
+
+                    with torch.cuda.stream(stream):
+                        run()
+                    torch.cuda.synchronize()
+                    row["verified"] &= checksum_ok(out.cpu().numpy(), csr, a, b, K, N, V)
+                    ms, _ = time_gpu(run, steps, warmup, stream)
+                    row["tops"][name] = 2.0 * csr.nnz_vec * V * N / (ms * 1e-3) / 1e12
+                row["a100_tops"] = {st: PUBLISHED_SPMM_ABLATION[(st, la, lb, s, V)][0]
+                                    for st in ("basic", "conflict_free", "prefetch", "shuffle")
+                                    if (st, la, lb, s, V) in PUBLISHED_SPMM_ABLATION}
+                cases.append(row)
+    return {"workload": "SpMM m_vec=256, K=2304, N=512, s in {0.7, 0.9}, V in {2, 8}, unsigned (Poisson mask)",
+            "a100_source": "plot/examples/spmm_abl_study.txt", "cases": cases}
 
 
 def sddmm_published_bench(steps, warmup, seed=43):
@@ -482,6 +565,160 @@ def serving_bench(n_requests=48, seed=202, budget_gib=4.0):
     return out
 
 
+def time_eager(run, stream, reps=20, warmup=3, rounds=3):
+    """Median per-call time (ms) of ``reps`` back-to-back eager calls on
+    ``stream``, bracketed by HIP events (library calls are not graph-captured)."""
+    import torch
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            run()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    times = []
+    for _ in range(rounds):
+        with torch.cuda.stream(stream):
+            start.record(stream)
+            for _ in range(reps):
+                run()
+            end.record(stream)
+        torch.cuda.synchronize()
+        times.append(start.elapsed_time(end) / reps)
+    return statistics.median(times)
+
+
+def vendor_bench(seed=7):
+    """The headline SpMM and SDDMM beside MI355X's vendor libraries, the roles
+    cuSPARSE and cuBLAS play in the reference's baselines
+    (baselines/spmm_benchmark.cpp:162-770, baselines/sddmm_benchmark.cpp:33-430).
+
+    Every case is timed the same way (eager calls between HIP events on one
+    stream).  ``speedup`` = vendor time / this library's time.  Integer vendor
+    results on the same matrix are compared with this library's output.
+    """
+    import torch
+    from magicube_amd import sddmm, spmm, synthetic, vendor
+
+    stream = torch.cuda.Stream()
+    M = K = 4096
+    N, V, s = 512, 8, 0.9
+    csr, a, b, a_packed, prep, rhs_np = make_workload(M, K, N, V, s, seed)
+    dev = spmm.DeviceSpmm.upload(prep)
+    rhs = torch.from_numpy(rhs_np).cuda()
+    ours_out = torch.empty((M, N), dtype=torch.int32, device="cuda")
+
+    def ours():
+        dev.matmul(rhs, N, out=ours_out, sign=spmm.SIGNED, stream=stream)
+
+    ours_ms = time_eager(ours, stream)
+    torch.cuda.synchronize()
+    ops = 2.0 * csr.nnz_vec * V * N
+    cases = []
+
+    def add(name, fmt, dtype, make, out=None, note=None):
+        entry = {"library": name, "format": fmt, "dtype": dtype}
+        try:
+            plan = make()
+        except Exception as exc:   # a combination the library does not provide
+            entry["error"] = str(exc)
+            cases.append(entry)
+            return
+        ms = time_eager(plan.run, stream)
+        entry.update({"ms": ms, "speedup": ms / ours_ms, "effective_tops": ops / (ms * 1e-3) / 1e12,
+                      "workspace_bytes": plan.workspace_bytes})
+        if out is not None:
+            torch.cuda.synchronize()
+            entry["matches_ours"] = bool(torch.equal(out, ours_out))
+        if note:
+            entry["note"] = note
+        plan.close()
+        cases.append(entry)
+
+    a8 = torch.from_numpy(a.astype(np.int8)).cuda()
+    b8 = torch.from_numpy(b.astype(np.int8).reshape(K, N)).cuda()
+    rp, ci, gather = vendor.scalar_csr(csr)
+    rp_t, ci_t = torch.from_numpy(rp).cuda(), torch.from_numpy(ci).cuda()
+    v8 = a8[torch.from_numpy(gather).cuda()].contiguous()
+    c32 = torch.empty((M, N), dtype=torch.int32, device="cuda")
+    for alg in vendor.CSR_ALGS:
+        add("rocSPARSE", f"CSR ({alg})", "int8", lambda alg=alg: vendor.csr_spmm(
+            vendor.I8, alg, M, K, N, rp_t, ci_t, v8, b8, c32, stream), out=c32)
+    brp, bci, bv = vendor.bsr_blocks(csr, a.astype(np.int8))
+    brp_t, bci_t, bv_t = (torch.from_numpy(x).cuda() for x in (brp, bci, bv))
+    add("rocSPARSE", f"BSR {V}x{V} (same matrix, vectors padded)", "int8", lambda: vendor.bsr_spmm(
+        vendor.I8, csr.m_vec, K // V, N, V, brp_t, bci_t, bv_t, b8, c32, stream), out=c32)
+    urp, uci = vendor.uniform_blocks(csr.m_vec, K, V, 1.0 - s, seed + 5)
+    uv = torch.from_numpy(synthetic.random_lanes(uci.size * V * V, 8, True, seed + 6).astype(np.int8)).cuda()
+    urp_t, uci_t = torch.from_numpy(urp).cuda(), torch.from_numpy(uci).cuda()
+    ublk = f"{uci.size // csr.m_vec} blocks per block row"
+    add("rocSPARSE", f"BSR {V}x{V} (uniform blocks, equal density)", "int8", lambda: vendor.bsr_spmm(
+        vendor.I8, csr.m_vec, K // V, N, V, urp_t, uci_t, uv, b8, c32, stream),
+        note=f"the reference's Blocked-ELL baseline structure, {ublk}")
+    dense_a = torch.from_numpy(vendor.dense_lhs(csr, a.astype(np.int8))).cuda()
+    b8_t = b8.t().contiguous()
+    add("hipBLASLt", "dense GEMM (NN)", "int8", lambda: vendor.gemm(vendor.I8, M, K, N, dense_a, b8, c32,
+                                                                  False, stream), out=c32)
+    add("hipBLASLt", "dense GEMM (TN)", "int8", lambda: vendor.gemm(vendor.I8, M, K, N, dense_a, b8_t, c32,
+                                                                  True, stream), out=c32)
+    dense_h = dense_a.to(torch.float16)
+    b16, b16_t = b8.to(torch.float16), b8_t.to(torch.float16)
+    c16 = torch.empty((M, N), dtype=torch.float16, device="cuda")
+    add("hipBLASLt", "dense GEMM (NN)", "fp16", lambda: vendor.gemm(vendor.F16, M, K, N, dense_h, b16, c16,
+                                                                  False, stream))
+    add("hipBLASLt", "dense GEMM (TN)", "fp16", lambda: vendor.gemm(vendor.F16, M, K, N, dense_h, b16_t, c16,
+                                                                  True, stream))
+    cf32 = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    v16 = v8.to(torch.float16)
+    for alg in vendor.CSR_ALGS:
+        add("rocSPARSE", f"CSR ({alg})", "fp16", lambda alg=alg: vendor.csr_spmm(
+            vendor.F16, alg, M, K, N, rp_t, ci_t, v16, b16, cf32, stream))
+    uv16 = uv.to(torch.float16)
+    add("rocSPARSE", f"BSR {V}x{V} (uniform blocks, equal density)", "fp16", lambda: vendor.bsr_spmm(
+        vendor.F16, csr.m_vec, K // V, N, V, urp_t, uci_t, uv16, b16, cf32, stream),
+        note=f"the reference's Blocked-ELL baseline structure, {ublk}")
+    spmm_res = {"workload": f"SpMM M=K={M}, N={N}, V={V}, s={s}, signed int8 (L8-R8)", "ours_ms": ours_ms,
+                "ours_tops": ops / (ours_ms * 1e-3) / 1e12, "cases": cases}
+    del dense_a, dense_h, v8, v16
+
+    # SDDMM: M = N = 4096, K = 64, V = 8, s = 0.9, signed int8
+    Ks = 64
+    mcsr = synthetic.uniform_block_mask(M, M, V, s, seed + 9)
+    mask = sddmm.DeviceMask.upload(mcsr)
+    la = synthetic.random_lanes(M * Ks, 8, True, seed + 10)
+    lb = synthetic.random_lanes(M * Ks, 8, True, seed + 11)
+    lhs = torch.from_numpy(la.astype(np.int8).view(np.int32)).cuda()
+    rhs_s = torch.from_numpy(lb.astype(np.int8).view(np.int32)).cuda()
+    sres = torch.zeros(mask.output_size(), dtype=torch.int32, device="cuda")
+
+    def ours_s():
+        mask.sddmm(8, lhs, rhs_s, Ks, out=sres, stream=stream)
+
+    ours_sms = time_eager(ours_s, stream)
+    sops = 2.0 * mcsr.nnz_vec * V * Ks
+    cases = []
+    ops = sops
+    ours_ms = ours_sms
+    srp, sci, _ = vendor.scalar_csr(mcsr)
+    srp_t, sci_t = torch.from_numpy(srp).cuda(), torch.from_numpy(sci).cuda()
+    l16 = torch.from_numpy(la.astype(np.float16).reshape(M, Ks)).cuda()
+    r16 = torch.from_numpy(lb.astype(np.float16).reshape(M, Ks)).cuda()
+    sv = torch.ones(sci.size, dtype=torch.float32, device="cuda")
+    add("rocSPARSE", "SDDMM CSR (same mask)", "fp16", lambda: vendor.csr_sddmm(M, M, Ks, srp_t, sci_t, l16, r16,
+                                                                             sv, stream))
+    l8 = torch.from_numpy(la.astype(np.int8).reshape(M, Ks)).cuda()
+    r8_t = torch.from_numpy(lb.astype(np.int8).reshape(M, Ks)).cuda()    # rhs is N x K: the TN form
+    dense_c = torch.empty((M, M), dtype=torch.int32, device="cuda")
+    add("hipBLASLt", "dense GEMM (TN), full M x N output", "int8",
+        lambda: vendor.gemm(vendor.I8, M, Ks, M, l8, r8_t, dense_c, True, stream))
+    dense_h16 = torch.empty((M, M), dtype=torch.float16, device="cuda")
+    add("hipBLASLt", "dense GEMM (TN), full M x N output", "fp16",
+        lambda: vendor.gemm(vendor.F16, M, Ks, M, l16, r16, dense_h16, True, stream))
+    sddmm_res = {"workload": f"SDDMM M=N={M}, K={Ks}, V={V}, s={s}, signed int8", "ours_ms": ours_sms,
+                 "ours_tops": sops / (ours_sms * 1e-3) / 1e12, "cases": cases}
+    return {"spmm": spmm_res, "sddmm": sddmm_res,
+            "note": "eager calls between HIP events for every entry; speedup = vendor ms / this library's ms"}
+
+
 def load_traffic(name):
     """HBM bytes per launch recorded in a profiles/ summary ("hbm_bytes_per_launch: N")."""
     path = os.path.join(HERE, "profiles", name)
@@ -519,6 +756,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-sddmm", action="store_true")
     ap.add_argument("--no-encoder", action="store_true", help="skip the 4-layer encoder benchmark")
+    ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE / hipBLASLt comparison")
+    ap.add_argument("--no-ablation", action="store_true", help="skip the SpMM ablation")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the every-pair sweep and the published-shape SDDMM")
     ap.add_argument("--config", type=int, default=None, help="explicit SpMM tile config index")
@@ -615,6 +854,12 @@ def main():
 
     if not args.no_encoder:
         result["encoder"] = encoder_bench()
+
+    if not args.no_vendor:
+        result["vendor"] = vendor_bench()
+
+    if not args.no_ablation:
+        result["ablation"] = ablation_bench(max(1, args.steps // 4), max(1, args.warmup // 4))
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(csr, a_packed, rhs_np, M, K, N, V)

@@ -2,8 +2,10 @@
 
 Compiles every translation unit under ``magicube_amd/csrc`` with hipcc for
 gfx950 (in parallel, into ``magicube_amd/_build``) and links them into the
-in-tree shared library ``magicube_amd/libmagicube_hip.so``; also builds the
-reference-compatible benchmark drivers when their sources are present.
+in-tree shared library ``magicube_amd/libmagicube_hip.so``; links the
+vendor-library comparators into ``magicube_amd/libmagicube_vendor.so`` and
+builds the reference-compatible benchmark drivers when their sources are
+present.
 """
 from __future__ import annotations
 
@@ -39,6 +41,11 @@ LIB_SOURCES = [
     "spmm_l16r4.hip",
 ]
 OPTIONAL_SOURCES = ["sddmm.hip", "attention.hip"]
+# Vendor-library comparators (rocSPARSE, hipBLASLt): a library of their own so
+# the product library carries no dependency on them.
+VENDOR_LIB = os.path.join(HERE, "libmagicube_vendor.so")
+VENDOR_SOURCES = ["vendor.cpp"]
+VENDOR_LIBS = ["-lrocsparse", "-lhipblaslt", "-lamdhip64"]
 DRIVERS = {"spmm_benchmark": "spmm_benchmark.cpp", "sddmm_benchmark": "sddmm_benchmark.cpp"}
 
 
@@ -68,6 +75,13 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    vobjs = [_compile(src) for src in VENDOR_SOURCES]
+    tmp = VENDOR_LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", "-o", tmp, *vobjs, f"-L{ROCM}/lib", f"-Wl,-rpath,{ROCM}/lib", *VENDOR_LIBS]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, VENDOR_LIB)
     for exe, src in DRIVERS.items():
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):

@@ -10,16 +10,18 @@
 namespace mc {
 
 // Tile configurations {lane_dwords, tiles_per_block, split_k, depth, kind,
-// waves_per_simd}: 0-7 and 12 gather kernel (four dwords per lane: 8- and
+// waves_per_simd}: 0-7 and 12-14 gather kernel (four dwords per lane: 8- and
 // 16-bit B); 8-11 B-stationary kernel (8-bit B), for which the fields are {B dwords per slice row, waves per workgroup,
 // vector rows per workgroup, prefetch depth}.  waves_per_simd > 1 makes the
-// register allocation fit that many waves per SIMD.
-constexpr int kNumSpmmConfigs = 13;
+// register allocation fit that many waves per SIMD.  13 and 14 are config 7
+// with one and two k-blocks of gathers in flight per wave instead of three
+// (the software-pipelining ablation).
+constexpr int kNumSpmmConfigs = 15;
 constexpr mc_tile_config kSpmmConfigs[kNumSpmmConfigs] = {
     {2, 4, 1, 2, 0, 0}, {2, 4, 2, 2, 0, 0}, {4, 2, 2, 2, 0, 0}, {4, 2, 4, 2, 0, 0},
     {1, 4, 2, 3, 0, 0}, {2, 2, 2, 2, 0, 0}, {4, 1, 4, 2, 0, 0}, {1, 8, 1, 3, 0, 0},
     {8, 16, 32, 4, 1, 0}, {8, 8, 32, 4, 1, 0}, {16, 16, 32, 4, 1, 0}, {8, 16, 16, 4, 1, 0},
-    {1, 4, 1, 2, 0, 6},
+    {1, 4, 1, 2, 0, 6}, {1, 8, 1, 1, 0, 0}, {1, 8, 1, 2, 0, 0},
 };
 
 // Whether a B-stationary config fits the shape (slice width divides n, the
@@ -74,6 +76,8 @@ mc_status spmm_launch_cfg(int cfg, const SpmmArgs &a, hipStream_t s) {
         if constexpr (BB == 8) return spmm_stationary_launch<V, AB, SGN, 32, 16, 4>(a, 16, s);
         else return MC_EUNSUPPORTED;
     case 12: return spmm_launch_one<V, AB, BB, SGN, 1, 4, 1, 2, 6>(a, s);
+    case 13: return spmm_launch_one<V, AB, BB, SGN, 1, 8, 1, 1>(a, s);
+    case 14: return spmm_launch_one<V, AB, BB, SGN, 1, 8, 1, 2>(a, s);
     default: return MC_EINVAL_ARG;
     }
 }

@@ -24,11 +24,14 @@ class SpmmCase:
     rhs_packed: np.ndarray  # int32 words, K * N * rhs_bits / 32
 
 
-def make_case(lhs_bits, rhs_bits, V, signed, M, K, N, sparsity, seed, mask="poisson") -> SpmmCase:
+def make_case(lhs_bits, rhs_bits, V, signed, M, K, N, sparsity, seed, mask="poisson", a=None, b=None) -> SpmmCase:
+    """Seeded case; ``a`` / ``b`` replace the generated lane values when given."""
     gen = synthetic.poisson_block_mask if mask == "poisson" else synthetic.uniform_block_mask
     csr = gen(M, K, V, sparsity, seed)
-    a = synthetic.random_lanes(csr.nnz_vec * V, lhs_bits, signed, seed + 1)
-    b = synthetic.random_lanes(K * N, rhs_bits, signed, seed + 2)
+    if a is None:
+        a = synthetic.random_lanes(csr.nnz_vec * V, lhs_bits, signed, seed + 1)
+    if b is None:
+        b = synthetic.random_lanes(K * N, rhs_bits, signed, seed + 2)
     prep = formats.prepare_spmm(csr, formats.pack_lanes(a, lhs_bits), lhs_bits, rhs_bits)
     rhs = formats.pack_lanes(b, rhs_bits)
     rhs = np.concatenate([rhs, np.zeros((-rhs.size) % 4, np.uint8)]).view(np.int32)

@@ -54,7 +54,7 @@ def test_spmm_small_exact(orc, pair, V, signed):
     assert np.array_equal(got, oracle_result(orc, case))
 
 
-GATHER_CONFIGS = [0, 1, 2, 3, 4, 5, 6, 7, 12]
+GATHER_CONFIGS = [0, 1, 2, 3, 4, 5, 6, 7, 12, 13, 14]
 
 
 @pytest.mark.parametrize("config", GATHER_CONFIGS)
