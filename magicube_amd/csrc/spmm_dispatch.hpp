@@ -10,18 +10,23 @@
 namespace mc {
 
 // Tile configurations {lane_dwords, tiles_per_block, split_k, depth, kind,
-// waves_per_simd}: 0-7 and 12-14 gather kernel (four dwords per lane: 8- and
+// waves_per_simd}: 0-7 and 12-23 gather kernel (four dwords per lane: 8- and
 // 16-bit B); 8-11 B-stationary kernel (8-bit B), for which the fields are {B dwords per slice row, waves per workgroup,
 // vector rows per workgroup, prefetch depth}.  waves_per_simd > 1 makes the
 // register allocation fit that many waves per SIMD.  13 and 14 are config 7
 // with one and two k-blocks of gathers in flight per wave instead of three
-// (the software-pipelining ablation).
-constexpr int kNumSpmmConfigs = 15;
+// (the software-pipelining ablation).  15-23 vary the waves per row (1, 2,
+// 4, 8), the split of a row's k-blocks over waves and the depth; the defaults
+// (15, 16, 17, 20, 22) come from them.
+constexpr int kNumSpmmConfigs = 24;
 constexpr mc_tile_config kSpmmConfigs[kNumSpmmConfigs] = {
     {2, 4, 1, 2, 0, 0}, {2, 4, 2, 2, 0, 0}, {4, 2, 2, 2, 0, 0}, {4, 2, 4, 2, 0, 0},
     {1, 4, 2, 3, 0, 0}, {2, 2, 2, 2, 0, 0}, {4, 1, 4, 2, 0, 0}, {1, 8, 1, 3, 0, 0},
     {8, 16, 32, 4, 1, 0}, {8, 8, 32, 4, 1, 0}, {16, 16, 32, 4, 1, 0}, {8, 16, 16, 4, 1, 0},
     {1, 4, 1, 2, 0, 6}, {1, 8, 1, 1, 0, 0}, {1, 8, 1, 2, 0, 0},
+    {1, 1, 1, 1, 0, 0}, {1, 2, 1, 1, 0, 0}, {1, 4, 1, 1, 0, 0}, {1, 2, 1, 2, 0, 0},
+    {1, 4, 1, 3, 0, 0}, {1, 4, 2, 1, 0, 0}, {1, 8, 2, 1, 0, 0}, {1, 2, 2, 1, 0, 0},
+    {1, 4, 2, 2, 0, 0},
 };
 
 // Whether a B-stationary config fits the shape (slice width divides n, the
@@ -78,6 +83,15 @@ mc_status spmm_launch_cfg(int cfg, const SpmmArgs &a, hipStream_t s) {
     case 12: return spmm_launch_one<V, AB, BB, SGN, 1, 4, 1, 2, 6>(a, s);
     case 13: return spmm_launch_one<V, AB, BB, SGN, 1, 8, 1, 1>(a, s);
     case 14: return spmm_launch_one<V, AB, BB, SGN, 1, 8, 1, 2>(a, s);
+    case 15: return spmm_launch_one<V, AB, BB, SGN, 1, 1, 1, 1>(a, s);
+    case 16: return spmm_launch_one<V, AB, BB, SGN, 1, 2, 1, 1>(a, s);
+    case 17: return spmm_launch_one<V, AB, BB, SGN, 1, 4, 1, 1>(a, s);
+    case 18: return spmm_launch_one<V, AB, BB, SGN, 1, 2, 1, 2>(a, s);
+    case 19: return spmm_launch_one<V, AB, BB, SGN, 1, 4, 1, 3>(a, s);
+    case 20: return spmm_launch_one<V, AB, BB, SGN, 1, 4, 2, 1>(a, s);
+    case 21: return spmm_launch_one<V, AB, BB, SGN, 1, 8, 2, 1>(a, s);
+    case 22: return spmm_launch_one<V, AB, BB, SGN, 1, 2, 2, 1>(a, s);
+    case 23: return spmm_launch_one<V, AB, BB, SGN, 1, 4, 2, 2>(a, s);
     default: return MC_EINVAL_ARG;
     }
 }

@@ -54,7 +54,7 @@ def test_spmm_small_exact(orc, pair, V, signed):
     assert np.array_equal(got, oracle_result(orc, case))
 
 
-GATHER_CONFIGS = [0, 1, 2, 3, 4, 5, 6, 7, 12, 13, 14]
+GATHER_CONFIGS = [0, 1, 2, 3, 4, 5, 6, 7, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23]
 
 
 @pytest.mark.parametrize("config", GATHER_CONFIGS)
