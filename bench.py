@@ -6,10 +6,10 @@ already resident in HBM): L8-R8 SpMM, A 4096 x 4096 vector-sparse with V=8,
 90% sparsity (Poisson nonzero count per 8-row block, seeded), signed
 full-range int8 values, dense B 4096 x 512 int8, int32 output C = A B.
 
-W untimed launches (default 5000, about 50 ms: the first milliseconds of
-work run below the GPU's working clocks) precede the timing.  The K timed
-steps are captured once into a HIP graph on the launch stream and replayed;
-HIP events on that stream bracket the replay.  TOPS follow the
+The K timed steps are captured once into a HIP graph on the launch stream
+and replayed; HIP events on that stream bracket the replay.  W untimed steps
+(default 5000, about 45 ms) run first as back-to-back replays of the same
+graph: the GPU reaches its working clocks only under sustained load.  TOPS follow the
 reference's formula 2 * nnz_vec * V * N / t on unpadded nonzeros
 (SpMM/SpMM/spmm_benchmark.cpp:112,637).  The roofline entry uses the
 compulsory bytes nnz_vec*(V + 4) + 4*(m_vec+1) + K*N + 4*M*N against
@@ -124,8 +124,16 @@ def checksum_ok(C, csr, a, b, K, N, V, seed=0):
 
 
 def time_gpu(run_once, steps, warmup, stream):
+    """Median per-step time (ms) of ``steps`` graph-replayed launches.
+
+    A few eager launches initialize the work; the ``steps`` launches are then
+    captured once into a HIP graph, the graph is replayed untimed until
+    ``warmup`` steps have run back to back (the GPU reaches its working clocks
+    only under sustained load, not between eager launches), and five timed
+    replays follow, each bracketed by HIP events on the capture stream."""
     import torch
-    for _ in range(warmup):
+    eager = min(warmup, 3)
+    for _ in range(eager):
         run_once()
     torch.cuda.synchronize()
     graph = torch.cuda.CUDAGraph()
@@ -139,7 +147,8 @@ def time_gpu(run_once, steps, warmup, stream):
     # replay() launches on the current stream: keep it the capture stream so
     # the events bracket exactly the replayed launches
     with torch.cuda.stream(stream):
-        graph.replay()      # untimed replay
+        for _ in range(max(1, -(-(warmup - eager) // steps))):
+            graph.replay()      # untimed
         torch.cuda.synchronize()
         for _ in range(reps):
             torch.cuda.synchronize()

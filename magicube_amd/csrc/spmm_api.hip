@@ -24,7 +24,7 @@ int default_config(int lhs_bits, int rhs_bits, int V, int n, int m_vec) {
     // columns), and when the launch would hold fewer than 2048 waves (8 per
     // CU) the row's k-blocks are also split between two waves.  Measured on
     // MI355X over M = K = 4096, N in {128, 256, 512}, V in {2, 8}, sparsity
-    // 0.7-0.98, every pair, and the reference's 2304-deep shapes: within 8 %
+    // 0.7-0.98, every pair, and the reference's 2304-deep shapes: within 9 %
     // of the best tile everywhere (the autotuner covers the rest).
     const int tile_cols = 16 * (32 / rhs_bits);
     const int tiles = (n + tile_cols - 1) / tile_cols;
