@@ -86,6 +86,49 @@ PUBLISHED_SPMM_ABLATION = {
     ("shuffle", 4, 4, 0.7, 2): (9.52411, 684), ("shuffle", 4, 4, 0.7, 8): (36.9262, 697),
     ("shuffle", 4, 4, 0.9, 2): (5.9758, 710), ("shuffle", 4, 4, 0.9, 8): (21.0911, 723),
 }
+# The reference's A100 runs on DLMC rn50 random_pruning bottleneck_2_block_group3_5_1
+# (256 vector rows, K = 2304), L8-R8 Magicube vs the int8 baselines, per-launch event timing:
+# (sparsity, V, N) -> (Magicube ms, cuBLAS int8 dense ms, cuSPARSE int8 Blocked-ELL ms,
+#                      and their lines in plot/examples/spmm_magicube_8b8b.txt,
+#                      spmm_cublas_int8.txt, spmm_cusparse_int8.txt).
+PUBLISHED_DLMC_A100 = {
+    (0.5, 2, 128): (0.0587884, 0.0362348, 0.132606, 539, 449, 539),
+    (0.7, 2, 128): (0.0394848, 0.035898, 0.0844892, 2075, 1729, 2075),
+    (0.8, 2, 128): (0.0322237, 0.0369709, 0.0625794, 3611, 3009, 3611),
+    (0.9, 2, 128): (0.0216988, 0.0356188, 0.0411213, 5147, 4289, 5147),
+    (0.95, 2, 128): (0.0158976, 0.0360255, 0.0285692, 6683, 5569, 6683),
+    (0.98, 2, 128): (0.0125451, 0.0380169, 0.0241786, 8219, 6849, 8219),
+    (0.5, 4, 128): (0.0619648, 0.111239, 0.132816, 9755, 8129, 9755),
+    (0.7, 4, 128): (0.0414849, 0.110935, 0.088539, 11291, 9409, 11291),
+    (0.8, 4, 128): (0.0311731, 0.111121, 0.0647134, 12827, 10689, 12827),
+    (0.9, 4, 128): (0.0208251, 0.111213, 0.0407606, 14363, 11969, 14363),
+    (0.95, 4, 128): (0.0158529, 0.111089, 0.0278881, 15899, 13249, 15899),
+    (0.98, 4, 128): (0.0132805, 0.112829, 0.024652, 17435, 14529, 17435),
+    (0.5, 8, 128): (0.0581013, 0.177379, 0.0536916, 18971, 15809, 18971),
+    (0.7, 8, 128): (0.0399071, 0.177774, 0.0363102, 20507, 17089, 20507),
+    (0.8, 8, 128): (0.0301272, 0.178686, 0.0267632, 22043, 18369, 22043),
+    (0.9, 8, 128): (0.0204869, 0.177367, 0.0205262, 23579, 19649, 23579),
+    (0.95, 8, 128): (0.0155595, 0.180883, 0.0165044, 25115, 20929, 25115),
+    (0.98, 8, 128): (0.0133504, 0.179176, 0.0150234, 26651, 22209, 26651),
+    (0.5, 2, 256): (0.0631769, 0.113565, 0.196471, 28187, 23489, 28187),
+    (0.7, 2, 256): (0.0421415, 0.113455, 0.16037, 29723, 24769, 29723),
+    (0.8, 2, 256): (0.0334617, 0.1136, 0.108045, 31259, 26049, 31259),
+    (0.9, 2, 256): (0.0241716, 0.1161, 0.0721037, 32795, 27329, 32795),
+    (0.95, 2, 256): (0.0165831, 0.113693, 0.0469088, 34331, 28609, 34331),
+    (0.98, 2, 256): (0.0137907, 0.115026, 0.0397939, 35867, 29889, 35867),
+    (0.5, 4, 256): (0.0662818, 0.178334, 0.196769, 37403, 31169, 37403),
+    (0.7, 4, 256): (0.0461313, 0.178086, 0.132066, 38939, 32449, 38939),
+    (0.8, 4, 256): (0.0353168, 0.179439, 0.116964, 40475, 33729, 40475),
+    (0.9, 4, 256): (0.0227795, 0.178115, 0.0755144, 42011, 35009, 42011),
+    (0.95, 4, 256): (0.019352, 0.181551, 0.0472593, 43547, 36289, 43547),
+    (0.98, 4, 256): (0.0141384, 0.178807, 0.0401977, 45083, 37569, 45083),
+    (0.5, 8, 256): (0.0631796, 0.116912, 0.0885132, 46619, 38849, 46619),
+    (0.7, 8, 256): (0.0430592, 0.11651, 0.0569483, 48155, 40129, 48155),
+    (0.8, 8, 256): (0.0357006, 0.115956, 0.0432874, 49691, 41409, 49691),
+    (0.9, 8, 256): (0.0226486, 0.117112, 0.0309084, 51227, 42689, 51227),
+    (0.95, 8, 256): (0.0173295, 0.116243, 0.0227704, 52763, 43969, 52763),
+    (0.98, 8, 256): (0.0147643, 0.1168, 0.0216203, 54299, 45249, 54299),
+}
 # SDDMM on A100 at the same matrix, N = 2304, K = 512 (basic kernel):
 # (bits, V) -> (TOPS, plot/examples/sddmm_abl_study.txt line)
 PUBLISHED_SDDMM_K512 = {
@@ -123,14 +166,15 @@ def checksum_ok(C, csr, a, b, K, N, V, seed=0):
     return bool(np.array_equal(got & 0xFFFFFFFF, exp.reshape(-1) & 0xFFFFFFFF))
 
 
-def time_gpu(run_once, steps, warmup, stream):
+def time_gpu(run_once, steps, warmup, stream, warm_s=0.02):
     """Median per-step time (ms) of ``steps`` graph-replayed launches.
 
     A few eager launches initialize the work; the ``steps`` launches are then
     captured once into a HIP graph, the graph is replayed untimed until
-    ``warmup`` steps have run back to back (the GPU reaches its working clocks
-    only under sustained load, not between eager launches), and five timed
-    replays follow, each bracketed by HIP events on the capture stream."""
+    ``warmup`` steps and at least ``warm_s`` seconds of back-to-back work
+    have run (the GPU reaches its working clocks only under sustained load,
+    not between eager launches or host-side setup), and five timed replays
+    follow, each bracketed by HIP events on the capture stream."""
     import torch
     eager = min(warmup, 3)
     for _ in range(eager):
@@ -147,8 +191,13 @@ def time_gpu(run_once, steps, warmup, stream):
     # replay() launches on the current stream: keep it the capture stream so
     # the events bracket exactly the replayed launches
     with torch.cuda.stream(stream):
-        for _ in range(max(1, -(-(warmup - eager) // steps))):
+        t0 = time.perf_counter()
+        done = 0
+        while done < max(1, -(-(warmup - eager) // steps)) or time.perf_counter() - t0 < warm_s:
             graph.replay()      # untimed
+            done += 1
+            if done % 8 == 0:
+                torch.cuda.synchronize()
         torch.cuda.synchronize()
         for _ in range(reps):
             torch.cuda.synchronize()
@@ -600,14 +649,29 @@ def time_eager(run, stream, reps=20, warmup=3, rounds=3):
     return statistics.median(times)
 
 
+def time_call(run, stream, steps=50):
+    """(ms per call, how) for a launch sequence: graph-replayed like the
+    headline when the calls can be captured (library calls included), else
+    eager calls between HIP events."""
+    try:
+        ms, _ = time_gpu(run, steps, SECTION_WARMUP, stream)
+        return ms, "graph"
+    except Exception:
+        import torch
+        torch.cuda.synchronize()
+        return time_eager(run, stream), "eager"
+
+
 def vendor_bench(seed=7):
     """The headline SpMM and SDDMM beside MI355X's vendor libraries, the roles
     cuSPARSE and cuBLAS play in the reference's baselines
     (baselines/spmm_benchmark.cpp:162-770, baselines/sddmm_benchmark.cpp:33-430).
 
-    Every case is timed the same way (eager calls between HIP events on one
-    stream).  ``speedup`` = vendor time / this library's time.  Integer vendor
-    results on the same matrix are compared with this library's output.
+    Every case is timed the same way: graph-replayed calls on one stream
+    (``time_call``; eager calls between HIP events where a library call cannot
+    be captured, marked in ``timing``).  ``speedup`` = vendor time / this
+    library's time.  Integer vendor results on the same matrix, read after
+    the timed replays, are compared with this library's output.
     """
     import torch
     from magicube_amd import sddmm, spmm, synthetic, vendor
@@ -623,7 +687,7 @@ def vendor_bench(seed=7):
     def ours():
         dev.matmul(rhs, N, out=ours_out, sign=spmm.SIGNED, stream=stream)
 
-    ours_ms = time_eager(ours, stream)
+    ours_ms, _ = time_call(ours, stream)
     torch.cuda.synchronize()
     ops = 2.0 * csr.nnz_vec * V * N
     cases = []
@@ -636,9 +700,9 @@ def vendor_bench(seed=7):
             entry["error"] = str(exc)
             cases.append(entry)
             return
-        ms = time_eager(plan.run, stream)
-        entry.update({"ms": ms, "speedup": ms / ours_ms, "effective_tops": ops / (ms * 1e-3) / 1e12,
-                      "workspace_bytes": plan.workspace_bytes})
+        ms, how = time_call(plan.run, stream)
+        entry.update({"ms": ms, "timing": how, "speedup": ms / ours_ms,
+                      "effective_tops": ops / (ms * 1e-3) / 1e12, "workspace_bytes": plan.workspace_bytes})
         if out is not None:
             torch.cuda.synchronize()
             entry["matches_ours"] = bool(torch.equal(out, ours_out))
@@ -706,7 +770,7 @@ def vendor_bench(seed=7):
     def ours_s():
         mask.sddmm(8, lhs, rhs_s, Ks, out=sres, stream=stream)
 
-    ours_sms = time_eager(ours_s, stream)
+    ours_sms, _ = time_call(ours_s, stream)
     sops = 2.0 * mcsr.nnz_vec * V * Ks
     cases = []
     ops = sops
@@ -729,7 +793,69 @@ def vendor_bench(seed=7):
     sddmm_res = {"workload": f"SDDMM M=N={M}, K={Ks}, V={V}, s={s}, signed int8", "ours_ms": ours_sms,
                  "ours_tops": sops / (ours_sms * 1e-3) / 1e12, "cases": cases}
     return {"spmm": spmm_res, "sddmm": sddmm_res,
-            "note": "eager calls between HIP events for every entry; speedup = vendor ms / this library's ms"}
+            "note": "graph-replayed calls for every entry unless timing says eager; "
+                    "speedup = vendor ms / this library's ms"}
+
+
+def vendor_sweep_bench(seed=61):
+    """Speedup over the vendor int8 baselines across sparsity, V and N at the
+    reference's DLMC matrix shape (256 vector rows, K = 2304, N in {128, 256},
+    unsigned L8-R8), beside the same ratios from the reference's A100 runs
+    (Magicube vs cuBLAS int8 dense and cuSPARSE int8 Blocked-ELL).  Every
+    entry is timed the same way (``time_call``: graph-replayed calls).  The
+    A100 figures time each launch between its own events, launch latency
+    included.
+    The dense GEMM's time depends on (M, K, N) only; BSR uses uniform V x V
+    blocks at the matrix's element density."""
+    import torch
+    from magicube_amd import spmm, synthetic, vendor
+    stream = torch.cuda.Stream()
+    K = 2304
+    dense_ms = {}
+    cases = []
+    for N in (128, 256):
+        for V in (2, 4, 8):
+            M = 256 * V
+            a8 = torch.from_numpy(synthetic.random_lanes(M * K, 8, True, seed).astype(np.int8).reshape(M, K)).cuda()
+            bt = torch.from_numpy(synthetic.random_lanes(K * N, 8, True, seed + 1).astype(np.int8).reshape(N, K)).cuda()
+            c = torch.empty((M, N), dtype=torch.int32, device="cuda")
+            plan = vendor.gemm(vendor.I8, M, K, N, a8, bt, c, True, stream)
+            dense_ms[(V, N)] = time_call(plan.run, stream)[0]
+            plan.close()
+            for s in (0.5, 0.7, 0.8, 0.9, 0.95, 0.98):
+                csr, a, b, _, prep, rhs_np = make_workload(M, K, N, V, s, seed + 2, 8, 8, signed=False)
+                dev = spmm.DeviceSpmm.upload(prep)
+                rhs = torch.from_numpy(rhs_np).cuda()
+                out = torch.empty((M, N), dtype=torch.int32, device="cuda")
+
+                def run():
+                    dev.matmul(rhs, N, out=out, sign=spmm.UNSIGNED, stream=stream)
+
+                with torch.cuda.stream(stream):
+                    run()
+                torch.cuda.synchronize()
+                ok = checksum_ok(out.cpu().numpy(), csr, a, b, K, N, V)
+                ours = time_call(run, stream)[0]
+                urp, uci = vendor.uniform_blocks(csr.m_vec, K, V, 1.0 - s, seed + 3)
+                uv = torch.from_numpy(synthetic.random_lanes(uci.size * V * V, 8, True, seed + 4).astype(np.int8)).cuda()
+                b8 = torch.from_numpy(b.astype(np.uint8).view(np.int8).reshape(K, N)).cuda()
+                bsr = vendor.bsr_spmm(vendor.I8, csr.m_vec, K // V, N, V, torch.from_numpy(urp).cuda(),
+                                      torch.from_numpy(uci).cuda(), uv, b8, out, stream)
+                bsr_ms = time_call(bsr.run, stream)[0]
+                bsr.close()
+                pub = PUBLISHED_DLMC_A100.get((s, V, N))
+                cases.append({
+                    "sparsity": s, "V": V, "N": N, "nnz_vec": csr.nnz_vec, "verified": ok,
+                    "ours_ms": ours, "hipblaslt_int8_ms": dense_ms[(V, N)], "rocsparse_bsr_int8_ms": bsr_ms,
+                    "speedup_vs_dense": dense_ms[(V, N)] / ours, "speedup_vs_bsr": bsr_ms / ours,
+                    "a100_magicube_ms": pub[0] if pub else None,
+                    "a100_speedup_vs_cublas_int8": pub[1] / pub[0] if pub else None,
+                    "a100_speedup_vs_cusparse_int8": pub[2] / pub[0] if pub else None,
+                    "speedup_vs_a100_magicube": pub[0] / ours if pub else None})
+    return {"workload": "SpMM L8-R8 unsigned, m_vec=256, K=2304, N in {128, 256}, V in {2, 4, 8}, "
+                        "s in {0.5 .. 0.98} (Poisson mask)",
+            "a100_source": "plot/examples/spmm_magicube_8b8b.txt, spmm_cublas_int8.txt, spmm_cusparse_int8.txt",
+            "cases": cases}
 
 
 def load_traffic(name):
@@ -871,6 +997,7 @@ def main():
 
     if not args.no_vendor:
         result["vendor"] = vendor_bench()
+        result["vendor_sweep"] = vendor_sweep_bench()
 
     if not args.no_ablation:
         result["ablation"] = ablation_bench(max(1, args.steps // 4), SECTION_WARMUP // 2)
