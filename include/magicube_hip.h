@@ -185,12 +185,18 @@ mc_status mc_sddmm_run(int bits, mc_signedness sign, int layout, int m_vec, int 
 #define MC_ROUND_TRUNC 0
 #define MC_ROUND_NEAREST 1
 
-/* One quantization job: count fp16 values -> int8 bytes or packed int4. */
+/* One quantization job: count fp16 values -> int8 bytes or packed int4.
+ * row_len == 0: x holds the count values contiguously.  Otherwise x holds
+ * rows of row_len values (a multiple of 8) row_stride values apart, e.g. one
+ * head's [L][64] slice of a [L][batch][3 * heads * 64] projection output;
+ * out is always contiguous. */
 typedef struct mc_quant_job {
     const void *x;
     long long count;
     float scale;
     void *out;
+    long long row_len;
+    long long row_stride;
 } mc_quant_job;
 /* Replaces sptrans bquantization (src/cuda/quantization.cpp:3-20): jobs is a
  * device array of njobs jobs; max_count bounds every job's count. */
@@ -231,6 +237,9 @@ typedef struct mc_attn_matrix {
     int *o_int;
     unsigned char *p_out;
     int *s_out;
+    void *o16;         /* non-null: fp16 output (round to nearest) at o16 +   */
+    int o16_stride;    /* row * o16_stride + column (a multiple of 4, o16     */
+    int o16_pad;       /* 8-byte aligned) instead of the fp32 o               */
 } mc_attn_matrix;
 /* Fused quantized sparse attention over many matrices in one persistent
  * launch.  items is a device array of n_items (matrix index, vector row)

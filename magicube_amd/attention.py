@@ -42,12 +42,34 @@ class _AttnMatrixC(ctypes.Structure):
                 ("col_indices", ctypes.c_void_p), ("q", ctypes.c_void_p), ("k", ctypes.c_void_p),
                 ("v", ctypes.c_void_p), ("alpha", ctypes.c_float), ("o_scale", ctypes.c_float),
                 ("o", ctypes.c_void_p), ("o_int", ctypes.c_void_p), ("p_out", ctypes.c_void_p),
-                ("s_out", ctypes.c_void_p)]
+                ("s_out", ctypes.c_void_p), ("o16", ctypes.c_void_p), ("o16_stride", ctypes.c_int),
+                ("o16_pad", ctypes.c_int)]
 
 
 class _QuantJobC(ctypes.Structure):
     _fields_ = [("x", ctypes.c_void_p), ("count", ctypes.c_longlong), ("scale", ctypes.c_float),
-                ("out", ctypes.c_void_p)]
+                ("out", ctypes.c_void_p), ("row_len", ctypes.c_longlong), ("row_stride", ctypes.c_longlong)]
+
+
+def _rows(x: torch.Tensor) -> tuple[int, int]:
+    """(row_len, row_stride) of a quantize job's source: (0, 0) when x is
+    contiguous, else the rows of a [L, 64] view with unit column stride."""
+    if x.is_contiguous():
+        return 0, 0
+    if x.dim() != 2 or x.stride(1) != 1 or x.shape[1] % 8 != 0:
+        raise ValueError("q, k, v must be contiguous or [L, 64] views with unit column stride")
+    return int(x.shape[1]), int(x.stride(0))
+
+
+def _as_input(x: torch.Tensor) -> torch.Tensor:
+    """A q / k / v operand as the quantize launch reads it: kept as the
+    caller's tensor when contiguous or a row-strided [L, 64] view, else
+    copied."""
+    try:
+        _rows(x)
+        return x
+    except ValueError:
+        return x.contiguous()
 
 
 def _bytes_tensor(raw: bytes, device) -> torch.Tensor:
@@ -150,16 +172,19 @@ class _Dev:
 
     def desc_bytes(self) -> bytes:
         ptr = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
+        half = self.o.dtype == torch.float16
         return bytes(_AttnMatrixC(self.L, self.m_vec, self.row_offsets.data_ptr(), self.col_indices.data_ptr(),
                                   self.qq.data_ptr(), self.kq.data_ptr(), self.vq.data_ptr(), self.alpha,
-                                  self.o_scale, self.o.data_ptr(), ptr(self.o_int), ptr(self.p_out),
-                                  ptr(self.s_out)))
+                                  self.o_scale, 0 if half else self.o.data_ptr(), ptr(self.o_int),
+                                  ptr(self.p_out), ptr(self.s_out), self.o.data_ptr() if half else 0,
+                                  int(self.o.stride(0)) if half else 0, 0))
 
     def quant_job_bytes(self) -> tuple[bytes, bytes]:
         """(q and k jobs, v job) of the batched quantize launches."""
-        qk = b"".join(bytes(_QuantJobC(x.data_ptr(), x.numel(), s, y.data_ptr()))
+        qk = b"".join(bytes(_QuantJobC(x.data_ptr(), x.numel(), s, y.data_ptr(), *_rows(x)))
                       for x, s, y in ((self.q16, self.scales[0], self.qq), (self.k16, self.scales[1], self.kq)))
-        return qk, bytes(_QuantJobC(self.v16.data_ptr(), self.v16.numel(), self.scales[2], self.vq.data_ptr()))
+        return qk, bytes(_QuantJobC(self.v16.data_ptr(), self.v16.numel(), self.scales[2], self.vq.data_ptr(),
+                                    *_rows(self.v16)))
 
 
 def working_set_bytes(L: int, qk_bits: int) -> int:
@@ -172,21 +197,27 @@ def working_set_bytes(L: int, qk_bits: int) -> int:
 def make_resident(m: AttentionMatrix, scales: tuple, qk_bits: int, V: int, p_scale: float, device,
                   debug: bool = False, out: torch.Tensor | None = None) -> _Dev:
     """Uploads and aligns the mask when it is a host mask, and allocates the
-    matrix's quantized operands and output (or writes into ``out``, a
-    contiguous fp32 [L, 64] device tensor)."""
+    matrix's quantized operands and output, or writes into ``out``: a
+    contiguous fp32 [L, 64] tensor, or an fp16 [L, 64] view with unit column
+    stride and a row stride that is a multiple of 4 (8-byte aligned).  q, k
+    and v may be row-strided [L, 64] views (read in place)."""
     mask = m.mask if isinstance(m.mask, ResidentMask) else ResidentMask.from_csr(m.mask, device)
     if m.q.shape[1] != HEAD_DIM or mask.V != V or mask.m_vec * V != m.L:
         raise ValueError("matrix shapes do not match the batch")
     qk_row = HEAD_DIM * qk_bits // 8
-    if out is not None and (tuple(out.shape) != (m.L, HEAD_DIM) or out.dtype != torch.float32
-                            or not out.is_contiguous()):
-        raise ValueError("out must be a contiguous float32 [L, 64] tensor")
+    if out is not None:
+        ok32 = out.dtype == torch.float32 and out.is_contiguous()
+        ok16 = (out.dtype == torch.float16 and out.stride(1) == 1 and out.stride(0) % 4 == 0
+                and out.data_ptr() % 8 == 0 and out.stride(0) < 2 ** 31)
+        if tuple(out.shape) != (m.L, HEAD_DIM) or not (ok32 or ok16):
+            raise ValueError("out must be a contiguous float32 [L, 64] tensor or an fp16 [L, 64] view with unit "
+                             "column stride, a row stride that is a multiple of 4 and 8-byte alignment")
     sq, sk, sv = (float(x) for x in scales)
     alpha = float(np.float32(1.0) / (np.float32(sq) * np.float32(sk) * np.float32(math.sqrt(HEAD_DIM))))
     o_scale = float(np.float32(1.0) / (np.float32(sv) * np.float32(p_scale)))
     d = _Dev(L=m.L, m_vec=mask.m_vec, nnz=mask.nnz_vec, aligned_num=mask.aligned_num, pairs=mask.pairs,
              row_offsets=mask.row_offsets, col_indices=mask.col_indices, row_indices=mask.row_indices,
-             row_nnz=mask.row_nnz, q16=m.q.contiguous(), k16=m.k.contiguous(), v16=m.v.contiguous(),
+             row_nnz=mask.row_nnz, q16=_as_input(m.q), k16=_as_input(m.k), v16=_as_input(m.v),
              qq=torch.empty(max(m.L * qk_row, 16), dtype=torch.uint8, device=device),
              kq=torch.empty(max(m.L * qk_row, 16), dtype=torch.uint8, device=device),
              vq=torch.empty(max(m.L * HEAD_DIM, 16), dtype=torch.uint8, device=device),
@@ -371,10 +402,14 @@ class AttentionBatch:
             w["p"] = torch.zeros(max(d.aligned_num * V, 16), dtype=torch.uint8, device=self.device)
             w["inv_l"] = torch.empty(max(d.m_vec * V, 1), dtype=torch.float32, device=self.device)
             w["o_int"] = torch.empty((d.L, HEAD_DIM), dtype=torch.int32, device=self.device)
+            # fp32 result of the pipeline when the matrix's output is fp16
+            w["o32"] = (d.o if d.o.dtype == torch.float32 else
+                        torch.empty((d.L, HEAD_DIM), dtype=torch.float32, device=self.device))
             for name, (x, s, y, b) in {"jq": (d.q16, d.scales[0], d.qq, self.qk_bits),
                                        "jk": (d.k16, d.scales[1], d.kq, self.qk_bits),
                                        "jv": (d.v16, d.scales[2], d.vq, 8)}.items():
-                w[name] = (_struct_tensor([_QuantJobC(x.data_ptr(), x.numel(), s, y.data_ptr())], self.device), b)
+                w[name] = (_struct_tensor([_QuantJobC(x.data_ptr(), x.numel(), s, y.data_ptr(), *_rows(x))],
+                                          self.device), b)
         return w
 
     def forward_plain(self, stream=None) -> list[torch.Tensor]:
@@ -399,9 +434,12 @@ class AttentionBatch:
                                        _dptr(d.row_offsets), _dptr(d.col_indices), _dptr(w["p"]), _dptr(d.vq),
                                        _dptr(w["o_int"]), st), "spmm")
             row_scale = _dptr(w["inv_l"]) if self.p_mode == P_ROW_SCALED else None
-            _lib.check(lib.mc_attn_dequant(_dptr(w["o_int"]), d.L, HEAD_DIM, d.o_scale, row_scale, _dptr(d.o), st),
-                       "dequant")
-        return [d.o for d in self.mats]
+            _lib.check(lib.mc_attn_dequant(_dptr(w["o_int"]), d.L, HEAD_DIM, d.o_scale, row_scale, _dptr(w["o32"]),
+                                           st), "dequant")
+            if d.o.dtype != torch.float32:
+                with torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext():
+                    d.o.copy_(w["o32"])          # fp16 output: same rounding as the fused kernel's store
+        return [d.work["o32"] if d.work else d.o for d in self.mats]
 
     def hbm_bytes(self) -> int:
         total = 0

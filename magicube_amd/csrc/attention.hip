@@ -24,6 +24,9 @@ struct AttnMatrix {
     int *o_int;
     unsigned char *p_out;
     int *s_out;
+    __half *o16;
+    int o16_stride;
+    int o16_pad;
 };
 static_assert(sizeof(AttnMatrix) == sizeof(mc_attn_matrix), "descriptor layout");
 
@@ -37,6 +40,8 @@ struct QuantJob {
     long long count;
     float scale;
     void *out;
+    long long row_len;      // 0: x is contiguous; else rows of row_len values,
+    long long row_stride;   // row_stride values apart (row_len % 8 == 0)
 };
 static_assert(sizeof(QuantJob) == sizeof(mc_quant_job), "job layout");
 
@@ -48,10 +53,13 @@ __global__ __launch_bounds__(256) void quantize_kernel(const QuantJob *jobs, int
     const long long groups = (jb.count + 7) / 8;
     for (long long gi = (long long)blockIdx.x * 256 + threadIdx.x; gi < groups; gi += (long long)gridDim.x * 256) {
         const long long base = gi * 8;
+        // the 8 values never straddle a source row (row_len % 8 == 0)
+        const __half *src = jb.row_len ? jb.x + (base / jb.row_len) * jb.row_stride + base % jb.row_len
+                                       : jb.x + base;
         int q[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            float f = base + e < jb.count ? __half2float(jb.x[base + e]) * jb.scale : 0.f;
+            float f = base + e < jb.count ? __half2float(src[e]) * jb.scale : 0.f;
             f = fminf(fmaxf(f, lo), hi);
             q[e] = rounding == 0 ? (int)f : (int)rintf(f);
         }
@@ -218,6 +226,8 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
         M.o_int = uniform(Mv.o_int);
         M.p_out = uniform(Mv.p_out);
         M.s_out = uniform(Mv.s_out);
+        M.o16 = uniform(Mv.o16);
+        M.o16_stride = uniform(Mv.o16_stride);
         const int i_row = uniform(item.y);
         const int start = uniform(const_load(M.row_offsets + 2 * i_row)),
                   end = uniform(const_load(M.row_offsets + 2 * i_row + 1));
@@ -384,7 +394,17 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
                 of.y = ((float)oacc[1][i] * M.o_scale) * rs;
                 of.z = ((float)oacc[2][i] * M.o_scale) * rs;
                 of.w = ((float)oacc[3][i] * M.o_scale) * rs;
-                *reinterpret_cast<float4 *>(M.o + orow) = of;
+                if (M.o16) {
+                    // fp16 (round to nearest even) straight into a strided
+                    // [L][row stride] layout, e.g. the [L][batch][heads * 64]
+                    // input of the output projection
+                    const __half2 h01 = __floats2half2_rn(of.x, of.y), h23 = __floats2half2_rn(of.z, of.w);
+                    const size_t hrow = (size_t)(i_row * V + 4 * g + i) * (size_t)M.o16_stride + 4 * c;
+                    *reinterpret_cast<v2i *>(M.o16 + hrow) =
+                        v2i{*reinterpret_cast<const int *>(&h01), *reinterpret_cast<const int *>(&h23)};
+                } else {
+                    *reinterpret_cast<float4 *>(M.o + orow) = of;
+                }
             }
         }
     }

@@ -93,15 +93,29 @@ def mask_from_reference(row_indices: torch.Tensor, row_offsets: torch.Tensor, co
 
 
 class _Plan:
-    """Per (shape, mask, precision) state of SpMultiheadAttention."""
+    """Per (shape, mask, precision) state of SpMultiheadAttention.
 
-    def __init__(self, L: int, n_mat: int, mask: ResidentMask, rhs_pre: int, device):
-        self.stage = torch.empty((3, n_mat, L, HEAD_DIM), dtype=torch.float16, device=device)
-        self.out = torch.empty((n_mat, L, HEAD_DIM), dtype=torch.float32, device=device)
-        mats = [AttentionMatrix(self.stage[0, i], self.stage[1, i], self.stage[2, i], mask) for i in range(n_mat)]
+    The in-projection writes ``qkv`` ([L][batch][3 * E] fp16); the quantize
+    launch reads each (sample, head)'s [L][64] slices of it in place, and the
+    fused kernel writes each head's output as fp16 straight into ``o``
+    ([L][batch][E], the output projection's input), so no permute copies run
+    between the projections and the attention kernels."""
+
+    def __init__(self, L: int, bsz: int, heads: int, mask: ResidentMask, rhs_pre: int, device):
+        E = heads * HEAD_DIM
+        self.qkv = torch.empty((L, bsz, 3 * E), dtype=torch.float16, device=device)
+        self.o = torch.empty((L, bsz, E), dtype=torch.float16, device=device)
+        q5 = self.qkv.view(L, bsz, 3, heads, HEAD_DIM)
+        o4 = self.o.view(L, bsz, heads, HEAD_DIM)
+        mats, outs = [], []
+        for b in range(bsz):
+            for h in range(heads):
+                mats.append(AttentionMatrix(q5[:, b, 0, h], q5[:, b, 1, h], q5[:, b, 2, h], mask))
+                outs.append(o4[:, b, h])
+        n_mat = bsz * heads
         self.batch = AttentionBatch(mats, qk_bits=rhs_pre, V=mask.V, p_mode=P_REFERENCE, p_scale=SCALE_SFMX,
                                     scales=[(SCALE_QKV, SCALE_QKV, SCALE_QKV)] * n_mat, rounding=ROUND_TRUNC,
-                                    device=device, outputs=[self.out[i] for i in range(n_mat)])
+                                    device=device, outputs=outs)
 
 
 class SpMultiheadAttention(nn.MultiheadAttention):
@@ -142,21 +156,27 @@ class SpMultiheadAttention(nn.MultiheadAttention):
                 ri = row_indices if row_indices is not None else torch.zeros(1, dtype=torch.int32,
                                                                                device=query.device)
                 mask = mask_from_reference(ri, row_offsets, column_indices, L, vec_length)
-            plan = _Plan(L, bsz * H, mask, rhs_pre, query.device)
+            plan = _Plan(L, bsz, H, mask, rhs_pre, query.device)
             self._plans[key_] = plan
 
-        # in-projection; stage Q, K, V as [3][sample * head][L][64] fp16
+        # in-projection into the plan's [L][batch][3E] buffer, read in place
+        # by the quantize launch
+        if query.dtype != torch.float16:
+            raise NotImplementedError("the quantized path takes fp16 activations")
+        w, b = self.in_proj_weight, self.in_proj_bias
+        qkv2 = plan.qkv.view(L * bsz, 3 * E)
         if query is key and key is value:
-            qkv = F.linear(query, self.in_proj_weight, self.in_proj_bias)
+            x2 = query.reshape(L * bsz, E)
+            if b is None:
+                torch.mm(x2, w.t(), out=qkv2)
+            else:
+                torch.addmm(b, x2, w.t(), out=qkv2)
         else:
-            w, b = self.in_proj_weight, self.in_proj_bias
-            qkv = torch.cat([F.linear(x, w[i * E:(i + 1) * E], None if b is None else b[i * E:(i + 1) * E])
-                             for i, x in enumerate((query, key, value))], dim=-1)
-        plan.stage.view(3, bsz, H, L, HEAD_DIM).copy_(qkv.view(L, bsz, 3, H, HEAD_DIM).permute(2, 1, 3, 0, 4))
+            for i, x in enumerate((query, key, value)):
+                part = F.linear(x.reshape(L * bsz, E), w[i * E:(i + 1) * E], None if b is None else b[i * E:(i + 1) * E])
+                qkv2[:, i * E:(i + 1) * E].copy_(part)
         plan.batch.forward()
-        o = torch.empty((L, bsz, E), dtype=query.dtype, device=query.device)
-        o.view(L, bsz, H, HEAD_DIM).copy_(plan.out.view(bsz, H, L, HEAD_DIM).permute(2, 0, 1, 3))
-        return F.linear(o, self.out_proj.weight, self.out_proj.bias), None
+        return F.linear(plan.o, self.out_proj.weight, self.out_proj.bias), None
 
 
 class PositionalEncoding(nn.Module):

@@ -65,7 +65,7 @@ def test_block_batch_device_bytes_decode_to_valid_operands():
     """Decodes the descriptor and quantize-job buffers a block batch uploads,
     in the C record layout, and checks every pointer the kernels follow lands
     inside a live tensor of the right size (regression: concatenating padded
-    records with numpy once repacked the 32-byte jobs to 28 bytes)."""
+    records with numpy once repacked the jobs without their tail padding)."""
     import numpy as np
     import torch
     from magicube_amd import synthetic
@@ -88,14 +88,17 @@ def test_block_batch_device_bytes_decode_to_valid_operands():
     for bits in (8, 4):
         bb = attention.BlockBatch([b for w, b in blocks if w == bits], device="cpu")
         raw = lambda t: t.numpy().tobytes()  # noqa: E731
-        assert bb.desc.numel() == 88 * bb.n_mats and bb.qk_jobs.numel() == 64 * bb.n_mats
-        assert bb.v_jobs.numel() == 32 * bb.n_mats
+        # 104-byte descriptors and 48-byte jobs (include/magicube_hip.h)
+        assert attention._ATTN_DT.itemsize == 104 and attention._JOB_DT.itemsize == 48
+        assert bb.desc.numel() == 104 * bb.n_mats and bb.qk_jobs.numel() == 2 * 48 * bb.n_mats
+        assert bb.v_jobs.numel() == 48 * bb.n_mats
         desc = np.frombuffer(raw(bb.desc), dtype=attention._ATTN_DT)
         for j in np.frombuffer(raw(bb.qk_jobs), dtype=attention._JOB_DT).tolist():
-            x, cnt, _, out = j
+            x, cnt, _, out, row_len, _ = j
             assert 0 < cnt <= bb.max_count and inside(x, 2 * cnt) and inside(out, cnt * bits // 8)
-        for x, cnt, _, out in np.frombuffer(raw(bb.v_jobs), dtype=attention._JOB_DT).tolist():
-            assert inside(x, 2 * cnt) and inside(out, cnt)
+            assert row_len == 0          # block inputs are contiguous
+        for x, cnt, _, out, row_len, _ in np.frombuffer(raw(bb.v_jobs), dtype=attention._JOB_DT).tolist():
+            assert inside(x, 2 * cnt) and inside(out, cnt) and row_len == 0
         items = [it for it in bb.items.numpy().tolist() if it[0] >= 0]
         assert len(set(map(tuple, items))) == len(items) == sum(int(d["m_vec"]) for d in desc)
         assert bb.n_items == bb.items.shape[0]
@@ -107,7 +110,7 @@ def test_block_batch_device_bytes_decode_to_valid_operands():
             s0, s1 = int(ro[2 * row]), int(ro[2 * row + 1])
             assert 0 <= s0 <= s1 <= ci.numel() and s0 % 16 == 0
             assert s1 == s0 or 0 <= int(ci[s0:s1].min()) and int(ci[s0:s1].max()) < L
-            assert inside(int(d["o"]) + row * 8 * 256, 8 * 256)
+            assert inside(int(d["o"]) + row * 8 * 256, 8 * 256) and int(d["o16"]) == 0
             for f, size in (("q", L * 8 * bits), ("k", L * 8 * bits), ("v", L * 64)):
                 assert inside(int(d[f]), size)
 

@@ -53,3 +53,44 @@ def test_module_argument_checks():
         mha(x, x, x, need_weights=False, key_padding_mask=torch.zeros(1, 64, dtype=torch.bool), **kw)
     with pytest.raises(ValueError):
         spattention.SpMultiheadAttention(embed_dim=96, num_heads=2)    # head dimension 48
+
+
+def test_plan_reads_projection_in_place_and_writes_heads_into_output():
+    """The module's launch records point each (sample, head) at its slices of
+    the [L][batch][3E] projection (row stride batch * 3E) and of the
+    [L][batch][E] output (row stride batch * E), so no permute copy runs."""
+    import numpy as np
+    from magicube_amd import attention
+    L, bsz, H = 128, 2, 3
+    E = 64 * H
+    cols, _, pairs, order, _ = spattention.static_random_mask_aligned(L // 8, L, 0.9, 16, seed=1, device="cpu")
+    mask = spattention.mask_from_reference(order, pairs, cols, L, 8)
+    plan = spattention._Plan(L, bsz, H, mask, 8, "cpu")
+    desc = np.frombuffer(plan.batch.desc.numpy().tobytes(), dtype=attention._ATTN_DT)
+    qk = np.frombuffer(plan.batch.qk_jobs.numpy().tobytes(), dtype=attention._JOB_DT).reshape(-1, 2)
+    vj = np.frombuffer(plan.batch.v_jobs.numpy().tobytes(), dtype=attention._JOB_DT)
+    for b in range(bsz):
+        for h in range(H):
+            i = b * H + h
+            assert int(desc["o"][i]) == 0 and int(desc["o16_stride"][i]) == bsz * E
+            assert int(desc["o16"][i]) - plan.o.data_ptr() == 2 * (b * E + 64 * h)
+            for s, job in enumerate((qk[i, 0], qk[i, 1], vj[i])):
+                assert int(job["row_len"]) == 64 and int(job["row_stride"]) == bsz * 3 * E
+                assert int(job["count"]) == L * 64
+                assert int(job["x"]) - plan.qkv.data_ptr() == 2 * (b * 3 * E + s * E + 64 * h)
+
+
+def test_make_resident_rejects_unaligned_fp16_outputs():
+    import torch
+    from magicube_amd import attention
+    L = 64
+    cols, _, pairs, order, _ = spattention.static_random_mask_aligned(L // 8, L, 0.9, 16, seed=2, device="cpu")
+    mask = spattention.mask_from_reference(order, pairs, cols, L, 8)
+    q = torch.zeros((L, 64), dtype=torch.float16)
+    m = attention.AttentionMatrix(q, q, q, mask)
+    bad = torch.zeros((L, 66), dtype=torch.float16)[:, :64]          # row stride 66: not a multiple of 4
+    with pytest.raises(ValueError):
+        attention.make_resident(m, (1.0, 1.0, 1.0), 8, 8, 32.0, "cpu", out=bad)
+    ok = torch.zeros((L, 128), dtype=torch.float16)[:, 64:]
+    d = attention.make_resident(m, (1.0, 1.0, 1.0), 8, 8, 32.0, "cpu", out=ok)
+    assert d.o.data_ptr() == ok.data_ptr()
