@@ -56,10 +56,24 @@ __global__ __launch_bounds__(256) void quantize_kernel(const QuantJob *jobs, int
         // the 8 values never straddle a source row (row_len % 8 == 0)
         const __half *src = jb.row_len ? jb.x + (base / jb.row_len) * jb.row_stride + base % jb.row_len
                                        : jb.x + base;
+        float x[8];
+        if (base + 8 <= jb.count && (reinterpret_cast<uintptr_t>(src) & 15u) == 0) {
+            // one 16-byte load for the 8 values
+            const v4i w = *reinterpret_cast<const v4i *>(src);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const __half2 h2 = *reinterpret_cast<const __half2 *>(&w[e]);
+                x[2 * e] = __low2float(h2);
+                x[2 * e + 1] = __high2float(h2);
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) x[e] = base + e < jb.count ? __half2float(src[e]) : 0.f;
+        }
         int q[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            float f = base + e < jb.count ? __half2float(src[e]) * jb.scale : 0.f;
+            float f = x[e] * jb.scale;
             f = fminf(fmaxf(f, lo), hi);
             q[e] = rounding == 0 ? (int)f : (int)rintf(f);
         }
