@@ -62,7 +62,8 @@ __global__ __launch_bounds__(256) void quantize_kernel(const QuantJob *jobs, int
             const v4i w = *reinterpret_cast<const v4i *>(src);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const __half2 h2 = *reinterpret_cast<const __half2 *>(&w[e]);
+                const int we = w[e];
+                const __half2 h2 = *reinterpret_cast<const __half2 *>(&we);
                 x[2 * e] = __low2float(h2);
                 x[2 * e + 1] = __high2float(h2);
             }
