@@ -47,6 +47,9 @@ VENDOR_LIB = os.path.join(HERE, "libmagicube_vendor.so")
 VENDOR_SOURCES = ["vendor.cpp"]
 VENDOR_LIBS = ["-lrocsparse", "-lhipblaslt", "-lamdhip64"]
 DRIVERS = {"spmm_benchmark": "spmm_benchmark.cpp", "sddmm_benchmark": "sddmm_benchmark.cpp"}
+# the reference's vendor-library baseline drivers, on rocSPARSE / hipBLASLt
+VENDOR_DRIVERS = {"baseline_spmm_benchmark": "baseline_spmm_benchmark.cpp",
+                  "baseline_sddmm_benchmark": "baseline_sddmm_benchmark.cpp"}
 
 
 def _compile(src: str) -> str:
@@ -82,14 +85,16 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, VENDOR_LIB)
-    for exe, src in DRIVERS.items():
+    drivers = [(exe, src, []) for exe, src in DRIVERS.items()]
+    drivers += [(exe, src, ["-lmagicube_vendor"]) for exe, src in VENDOR_DRIVERS.items()]
+    for exe, src, extra in drivers:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         out = os.path.join(HERE, exe)
         cmd = [HIPCC, "-O3", "-std=c++17", "-x", "c++", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
-               path, "-o", out, f"-L{HERE}", "-lmagicube_hip", "-Wl,-rpath,$ORIGIN", f"-L{ROCM}/lib",
-               "-lamdhip64"]
+               path, "-o", out, f"-L{HERE}", *extra, "-lmagicube_hip", "-Wl,-rpath,$ORIGIN", f"-L{ROCM}/lib",
+               f"-Wl,-rpath,{ROCM}/lib", "-lamdhip64"]
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:
             raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

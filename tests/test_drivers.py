@@ -92,3 +92,18 @@ def test_sddmm_driver_synthetic_header():
     assert m and int(m.group(1)) % 8 == 0
     m2 = re.search(r" nonzero_vec: (\d+) aligned_ nonzero_vec: (\d+)", out)
     assert m2 and int(m2.group(1)) * 8 == int(m.group(1)) and int(m2.group(2)) % 8 == 0
+
+
+def test_baseline_drivers_usage_and_unsupported_modes():
+    """The vendor-baseline drivers print usage, and reject the reference's own
+    vectorSparse modes before any device call."""
+    for exe, argv in (("baseline_spmm_benchmark", ["synthetic:64,128,0.8,3", "64", "8", "0", "1", "0", "1", "1"]),
+                      ("baseline_sddmm_benchmark", ["synthetic:64,128,0.8,3", "64", "8", "0", "2", "1", "0", "1",
+                                                    "1"])):
+        path = os.path.join(PKG, exe)
+        _need(path)
+        r = subprocess.run([path, "--help"], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0 and "usage" in r.stdout
+        r = subprocess.run([path, *argv], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and "not carried" in r.stderr, r.stdout + r.stderr
+        assert "Sparse matrix: synthetic:64,128,0.8,3" in r.stdout

@@ -35,3 +35,26 @@ def test_sddmm_driver_passes(bits, K, V):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "SDDMM results verification: PASS" in r.stdout, r.stdout
     assert re.search(r"Magicube SDDMM runtime [0-9.e-]+ ms", r.stdout)
+
+
+# the reference's launch_spmm_*.py / launch_sddmm_*.py modes (sparse, kernel, mixed)
+@pytest.mark.parametrize("sparse,kernel,V", [(-1, 0, 8), (-2, 0, 4), (3, 0, 8), (3, 0, 2), (2, 0, 8), (1, 3, 4)])
+def test_baseline_spmm_driver_verifies(sparse, kernel, V):
+    cmd = [os.path.join(PKG, "baseline_spmm_benchmark"), "synthetic:256,512,0.8,11", "128", str(V), str(kernel),
+           "1", "1", str(sparse), "1"]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Results verified: they agree." in r.stdout, r.stdout
+    assert re.search(r"runtime [0-9.e-]+ ms", r.stdout)
+
+
+@pytest.mark.parametrize("sparse,kernel,V", [(0, 0, 8), (-1, 0, 4), (1, 3, 8)])
+def test_baseline_sddmm_driver_verifies(sparse, kernel, V):
+    cmd = [os.path.join(PKG, "baseline_sddmm_benchmark"), "synthetic:256,384,0.8,13", "64", str(V), str(kernel),
+           "2", "1", "1", str(sparse), "1"]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Results verified: they agree." in r.stdout, r.stdout
+    assert re.search(r"runtime [0-9.e-]+ ms", r.stdout)
