@@ -268,14 +268,33 @@ __global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
                     }
                     const int j = c0 + (tg + t) * 16 + r;
                     if (g < (V + 3) / 4 && tg + t < ctiles && j < wend) {
+                        auto store_each = [&]() {
 #pragma unroll
-                        for (int i = 0; i < HOME; ++i) {
-                            const int v = 4 * g + i;
-                            const int val = j < end ? res[i] : 0;
-                            const size_t o =
-                                OUT_REF ? (size_t)(j >> 3) * 8 * V + 8 * v + (j & 7) : (size_t)j * V + v;
-                            // later passes add to what this lane stored before
-                            args.out[o] = ps == 0 ? val : (int)((u32)args.out[o] + (u32)val);
+                            for (int i = 0; i < HOME; ++i) {
+                                const int v = 4 * g + i;
+                                const int val = j < end ? res[i] : 0;
+                                const size_t o =
+                                    OUT_REF ? (size_t)(j >> 3) * 8 * V + 8 * v + (j & 7) : (size_t)j * V + v;
+                                // later passes add to what this lane stored before
+                                args.out[o] = ps == 0 ? val : (int)((u32)args.out[o] + (u32)val);
+                            }
+                        };
+                        if constexpr (!OUT_REF && HOME == 4) {
+                            if ((reinterpret_cast<uintptr_t>(args.out) & 15u) == 0) {
+                                // [slot][V]: the lane's rows 4g..4g+3 are one 16-byte run
+                                v4i val = j < end ? v4i{res[0], res[1], res[2], res[3]} : v4i{0, 0, 0, 0};
+                                v4i *o = reinterpret_cast<v4i *>(args.out + (size_t)j * V + 4 * g);
+                                if (ps != 0) {
+                                    const v4i prev = *o;
+#pragma unroll
+                                    for (int i = 0; i < 4; ++i) val[i] = (int)((u32)prev[i] + (u32)val[i]);
+                                }
+                                *o = val;
+                            } else {
+                                store_each();
+                            }
+                        } else {
+                            store_each();
                         }
                     }
                 }
