@@ -123,9 +123,11 @@ typedef struct mc_tile_config {
 } mc_tile_config;
 int mc_spmm_num_tile_configs(void);
 mc_status mc_spmm_get_tile_config(int index, mc_tile_config *out);
-/* Generic SpMM with an explicit config index (-1: the table default for the
- * shape).  (lhs_bits, rhs_bits) in {(8,8), (12,8), (16,8), (16,16), (4,4),
- * (8,4), (12,4), (16,4)}. */
+/* Generic SpMM with an explicit config index (-1: the default for the shape:
+ * one dword of B per lane, up to four waves per row as n allows, and the
+ * row's k-blocks split over two waves when the launch would hold fewer than
+ * 2048 waves).  (lhs_bits, rhs_bits) in {(8,8), (12,8), (16,8), (16,16),
+ * (4,4), (8,4), (12,4), (16,4)}. */
 mc_status mc_spmm_run(int lhs_bits, int rhs_bits, mc_signedness sign, int config, int m_vec,
                       int vec_length, int n, int k, const int *row_indices,
                       const int *row_offsets, const int *column_indices, const int *values,
