@@ -909,7 +909,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the SpMM shards by vector-row blocks with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(the SpMM would shard by vector-row blocks with B replicated and no exchange)", file=sys.stderr)
         sys.exit(2)
 
     import torch
