@@ -17,7 +17,7 @@ MI355X's 8 TB/s HBM3E.  The CPU baseline times the restated host loop
 (compute_ref_integers) on the same arrays, single-threaded and with
 OpenMP, on this machine's host cores.
 
-Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: the benchmark measures
 one GPU.
 """
 from __future__ import annotations
@@ -908,7 +908,7 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the SpMM would shard by vector-row blocks with B replicated and no exchange)", file=sys.stderr)
         sys.exit(2)
 

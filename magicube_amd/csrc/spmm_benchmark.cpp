@@ -7,7 +7,7 @@
 //
 // bm is a DLMC .smtx file or "synthetic:M,K,sparsity[,seed]".  kernel 0 and
 // sparse 1 select the SpMM (the reference's other kernels are baselines that
-// this build does not carry); pairs (preA, preB) in {(8,8), (12,8), (16,8),
+// the library does not carry); pairs (preA, preB) in {(8,8), (12,8), (16,8),
 // (16,16), (4,4), (8,4), (12,4), (16,4)}, 12-bit lanes in 16-bit containers
 // as the reference holds them.  Operand values follow the reference's
 // generator (MakeDenseMatrix of the reference's TypeA for the pair), B is K x n, the
