@@ -170,3 +170,53 @@ def test_spmm_extreme_values(orc, pair, signed):
     rhs = formats.pack_lanes(case.b, lb)
     case.rhs_packed = np.concatenate([rhs, np.zeros((-rhs.size) % 4, np.uint8)]).view(np.int32)
     assert np.array_equal(run_case(case), oracle_result(orc, case))
+
+
+def _default_cfg(lhs, rhs, V, n, m_vec):
+    """The library's default tile rule, restated (spmm_api.hip default_config)."""
+    tc = 16 * (32 // rhs)
+    tiles = -(-n // tc)
+    tpb = 4 if tiles >= 4 else (2 if tiles >= 2 else 1)
+    if m_vec * (-(-tiles // tpb)) * tpb < 2048:
+        return 20 if tpb == 4 else 22
+    return {4: 17, 2: 16, 1: 15}[tpb]
+
+
+def _default_cases():
+    """Seeded shapes, eight per default tile (15, 16, 17, 20, 22): ragged N
+    (any multiple of 32/rhs_bits), row counts on both sides of the 2048-wave
+    split, every pair and V."""
+    rng = np.random.default_rng(2024)
+    cases = []
+    for i in range(40):
+        target = (15, 16, 17, 20, 22)[i % 5]
+        pair = PAIRS[(i // 5) % len(PAIRS)]
+        V = (2, 4, 8)[i % 3]
+        cpd = 32 // pair[1]
+        r = int(rng.integers(1, 17) if target == 15 else
+                rng.integers(17, 49) if target in (16, 22) else rng.integers(49, 97))
+        n = cpd * r
+        tiles = -(-r // 16)
+        tpb = 4 if tiles >= 4 else (2 if tiles >= 2 else 1)
+        per_row = -(-tiles // tpb) * tpb
+        if target in (20, 22):
+            m_vec = int(rng.integers(1, 2048 // per_row))
+        else:
+            m_vec = -(-2048 // per_row) + int(rng.integers(0, 200))
+        K = int(rng.integers(40, 600))
+        s = float(rng.choice([0.5, 0.8, 0.95]))
+        cases.append((pair, V, m_vec, K, n, s, i % 2 == 0, 1000 + i))
+    return cases
+
+
+def test_default_rule_cases_cover_every_default_tile():
+    got = {_default_cfg(*c[0], c[1], c[4], c[2]) for c in _default_cases()}
+    assert got == {15, 16, 17, 20, 22}, got
+
+
+@pytest.mark.parametrize("case", _default_cases(), ids=lambda c: f"L{c[0][0]}R{c[0][1]}-V{c[1]}-m{c[2]}-K{c[3]}-N{c[4]}-{'s' if c[6] else 'u'}")
+def test_spmm_default_tiles_exact(orc, case):
+    (la, lb), V, m_vec, K, N, s, signed, seed = case
+    c = make_case(la, lb, V, signed, M=m_vec * V, K=K, N=N, sparsity=s, seed=seed)
+    got = run_case(c)
+    assert np.array_equal(got, oracle_result(orc, c)), f"default tile {_default_cfg(la, lb, V, N, m_vec)}"
