@@ -224,7 +224,9 @@ mc_status mc_attn_dequant(const int *in, int rows, int cols, float scale, const 
 /* One attention matrix of the fused batched launch (device-resident
  * descriptor).  q, k: [L][64] int8 (or int4 packed [L][32] bytes); v: [L][64]
  * int8; o: [L][64] float.  o_int (int32 P Vq), p_out (P in the SpMM A layout)
- * and s_out (int32 S, [slot][V]) are optional debug outputs. */
+ * and s_out (int32 S, [slot][V]) are optional debug outputs.  L <= 2^24 (the
+ * kernel addresses rows with 32-bit byte offsets and marks masked slots with
+ * offsets >= 2^31). */
 typedef struct mc_attn_matrix {
     int L;
     int m_vec;

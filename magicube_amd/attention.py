@@ -30,6 +30,9 @@ from . import _lib, formats
 from .spmm import _dptr, _stream
 
 HEAD_DIM = 64
+# the kernels address Q/K/V rows with 32-bit byte offsets and mark masked
+# slots with offsets >= 2^31
+MAX_SEQ_LEN = 1 << 24
 P_REFERENCE = 0
 P_ROW_SCALED = 1
 ROUND_TRUNC = 0
@@ -204,6 +207,8 @@ def make_resident(m: AttentionMatrix, scales: tuple, qk_bits: int, V: int, p_sca
     mask = m.mask if isinstance(m.mask, ResidentMask) else ResidentMask.from_csr(m.mask, device)
     if m.q.shape[1] != HEAD_DIM or mask.V != V or mask.m_vec * V != m.L:
         raise ValueError("matrix shapes do not match the batch")
+    if m.L > MAX_SEQ_LEN:
+        raise ValueError(f"sequence length {m.L} exceeds {MAX_SEQ_LEN}")
     qk_row = HEAD_DIM * qk_bits // 8
     if out is not None:
         ok32 = out.dtype == torch.float32 and out.is_contiguous()

@@ -185,7 +185,7 @@ __global__ __launch_bounds__(64 * WPB) void softmax_requant_kernel(int m_vec, co
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 pq[t][i] = i < HOME ? quantize_prob(s[t][i], m[i < HOME ? i : 0], inv_l[i < HOME ? i : 0], p_scale, p_mode) : 0;
-        const v4i a = p_to_a_layout<HOME>(tiles[wave], pq, lane, home);
+        const v4i a = p_to_a_layout<HOME>(tiles[wave], pq, lane, home, 4 * g);
         const int slot0 = start + 64 * ch + 16 * g;        // first slot of this lane's 16-slot group
         if (c < V && slot0 < aend)
             *reinterpret_cast<v4i *>(p_out + (size_t)(slot0 >> 4) * 16 * V + 16 * c) = a;
@@ -215,12 +215,22 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
                                                              float p_scale, int p_mode) {
     constexpr int HOME = V < 4 ? V : 4;
     constexpr int QROW = kHeadDim * QKB / 8;          // bytes per Q/K row
+    constexpr int KCB = QROW / 4;                     // bytes of a row per lane in a K gather
     __shared__ __attribute__((aligned(16))) unsigned char tiles[WPB][16 * 64];
+    __shared__ __attribute__((aligned(16))) unsigned char ktiles[WPB][64 * QROW];
     // the wave id in an SGPR keeps work-item indices, descriptors and buffer
     // resources wave-uniform (no per-lane waterfall around each buffer load)
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c = lane & 15, g = lane >> 4;
     const bool home = g < (V + 3) / 4;
+    // Softmax rows per lane: the MFMA C tile holds the V real rows in lane
+    // groups g < (V + 3) / 4 (rows 4g + i); permlane swaps deal them to all
+    // four groups first, so that the per-logit work runs on real rows only.
+    // V = 8: group g holds rows row0 + {0, 1}, row0 = 4 (g & 1) + 2 (g >> 1);
+    // V = 4: group g holds row g; V = 2: groups 0 and 1 hold rows 0 and 1.
+    constexpr int HC = V == 8 ? 2 : 1;
+    const bool chome = V == 2 ? g < 2 : true;
+    const int row0 = V == 8 ? 4 * (g & 1) + 2 * (g >> 1) : g;
     const int nwaves = gridDim.x * WPB;
 
     for (int it = blockIdx.x * WPB + wave; it < n_items; it += nwaves) {
@@ -270,52 +280,113 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
         // are fully masked: they leave (m, l) bitwise unchanged and add P = 0,
         // so the loops run in pairs without branches.
         const __amdgpu_buffer_rsrc_t rc = make_rsrc(M.col_indices, 0x7FFFFFFFu);
+        // per lane: byte offset of its slot's V row (index * 64), or the
+        // out-of-range marker for slots past the row end (L * 64 < 2^31)
+        constexpr u32 kRowOob = 0x80000000u;
         auto load_idx = [&](int ch) {
             const int slot = start + 64 * ch + lane;
-            const int v = (int)buf_load_b32(rc, slot < end ? (u32)slot * 4u : 0x80000000u);
-            return slot < end ? v : -1;
+            const u32 v = buf_load_b32(rc, slot < end ? (u32)slot * 4u : kRowOob);
+            return slot < end ? v * (u32)kHeadDim : kRowOob;
         };
-        auto load_krows = [&](int my_idx, v4i (&kop)[4]) {
+        // K rows of slots 16t + lane/4, bytes [KCB * (lane % 4), +KCB): four
+        // consecutive lanes read one row, so a gather touches 16 rows per
+        // instruction once each (the MFMA layout would spread each row over
+        // lanes 16 apart and cost four L1 accesses per row segment)
+        auto load_krows = [&](u32 my_off, v4i (&kraw)[4]) {
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                const int idx = __shfl(my_idx, 16 * t + c, 64);
-                const u32 off = idx >= 0 ? (u32)idx * QROW + (u32)(16 * g * QKB / 8) : 0x80000000u;
-                kop[t] = load_k16<QKB>(rk, off);
+                u32 row = (u32)__shfl((int)my_off, 16 * t + (lane >> 2), 64);
+                if constexpr (QKB == 4) row = (row >> 1) | (row & kRowOob);   // 32-byte rows, marker kept
+                kraw[t] = load_k16<QKB>(rk, row + (u32)(KCB * (lane & 3)));
             }
+        };
+        // to the MFMA B layout (lane (c, g): row 16t + c, bytes [KCB * g, +KCB))
+        // through the wave's 64-row K tile; the KCB-byte pieces of a row are
+        // XOR-swizzled by row so that neither side has bank conflicts
+        auto k_to_b_layout = [&](const v4i (&kraw)[4], v4i (&kop)[4]) {
+            unsigned char *kt = ktiles[wave];
+            auto swz = [](int r) { return QKB == 8 ? (r >> 2) & 3 : (r >> 3) & 1; };
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int r = 16 * t + (lane >> 2);
+                unsigned char *dst = kt + r * QROW + KCB * ((lane & 3) ^ swz(r));
+                if constexpr (QKB == 8) *reinterpret_cast<v4i *>(dst) = kraw[t];
+                else *reinterpret_cast<v2i *>(dst) = v2i{kraw[t][0], kraw[t][1]};
+            }
+            // private to the wave, and one wave's LDS operations complete in
+            // order: wavefront-scope fences only keep the compiler from
+            // reordering the reads and writes around each other
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int r = 16 * t + c;
+                const unsigned char *src = kt + r * QROW + KCB * (g ^ swz(r));
+                if constexpr (QKB == 8) {
+                    kop[t] = *reinterpret_cast<const v4i *>(src);
+                } else {
+                    const v2i x = *reinterpret_cast<const v2i *>(src);
+                    kop[t] = v4i{x[0], x[1], 0, 0};
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         };
         // V rows of slots 16g + p (p = 0..15), dword c (columns 4c..4c+3)
-        auto load_vrows = [&](int my_idx, u32 (&raw)[16]) {
+        auto load_vrows = [&](u32 my_off, u32 (&raw)[16]) {
 #pragma unroll
             for (int p = 0; p < 16; ++p) {
-                const int idx = __shfl(my_idx, 16 * g + p, 64);
-                const u32 off = idx >= 0 ? (u32)idx * kHeadDim + 4u * c : 0x80000000u;
-                raw[p] = buf_load_b32(rv, off);
+                const u32 row = (u32)__shfl((int)my_off, 16 * g + p, 64);
+                raw[p] = buf_load_b32(rv, row + 4u * c);
             }
         };
-        // logits of one chunk in the MFMA C layout, scaled; masked -> -inf
-        auto logits = [&](int ch, const v4i (&kop)[4], float (&s)[4][4], int (&si)[4][4]) {
+        // C-tile rows 4g + i of one 16-slot tile -> this lane's rows row0 + j
+        auto compact_rows = [&](const v4i &acc, int (&out)[HC]) {
+            if constexpr (V == 8) {
+                // lanes 32..63 take rows 4g' + 2, 4g' + 3 of lanes 0..31 (g' = g - 2)
+                out[0] = (int)__builtin_amdgcn_permlane32_swap((u32)acc[0], (u32)acc[2], false, false)[0];
+                out[1] = (int)__builtin_amdgcn_permlane32_swap((u32)acc[1], (u32)acc[3], false, false)[0];
+            } else if constexpr (V == 4) {
+                // rows 0, 1 to groups 0, 1 and rows 2, 3 to groups 2, 3
+                const u32 a = __builtin_amdgcn_permlane16_swap((u32)acc[0], (u32)acc[1], false, false)[0];
+                const u32 b = __builtin_amdgcn_permlane16_swap((u32)acc[2], (u32)acc[3], false, false)[0];
+                out[0] = (int)__builtin_amdgcn_permlane32_swap(a, b, false, false)[0];
+            } else {
+                out[0] = (int)__builtin_amdgcn_permlane16_swap((u32)acc[0], (u32)acc[1], false, false)[0];
+            }
+        };
+        // logits of one chunk, scaled, in this lane's rows; masked -> -inf.
+        // si: the raw C tile (rows 4g + i), kept for the debug outputs only
+        auto logits = [&](int ch, const v4i (&kraw)[4], float (&s)[4][HC], int (&si)[4][4]) {
+            v4i kop[4];
+            k_to_b_layout(kraw, kop);
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 v4i kk = kop[t];
                 if constexpr (QKB == 4) kk = expand_nibbles(kk, true, 0u);
                 const v4i acc = mfma_i8(qop, kk, v4i{0, 0, 0, 0});
                 const int slot = start + 64 * ch + 16 * t + c;
+                if constexpr (DBG) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    si[t][i] = acc[i];
-                    s[t][i] = (home && i < HOME && slot < end) ? (float)acc[i] * M.alpha : -INFINITY;
+                    for (int i = 0; i < 4; ++i) si[t][i] = acc[i];
                 }
+                int v[HC];
+                compact_rows(acc, v);
+#pragma unroll
+                for (int j = 0; j < HC; ++j) s[t][j] = (chome && slot < end) ? (float)v[j] * M.alpha : -INFINITY;
             }
         };
 
-        float m[HOME], l[HOME];
+        float m[HC], l[HC];
 #pragma unroll
-        for (int i = 0; i < HOME; ++i) { m[i] = -INFINITY; l[i] = 0.f; }
+        for (int j = 0; j < HC; ++j) { m[j] = -INFINITY; l[j] = 0.f; }
         auto pass1_chunk = [&](int ch, const v4i (&kop)[4]) {
-            float s[4][4];
+            float s[4][HC];
             int si[4][4];
             logits(ch, kop, s, si);
-            softmax_chunk_update<HOME>(s, m, l);
+            softmax_chunk_update<HC>(s, m, l);
             if (DBG && M.s_out && home) {
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
@@ -328,8 +399,8 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
         };
         {
             v4i ka[4], kb[4];
-            int ia = load_idx(0);
-            int ib = load_idx(1);
+            u32 ia = load_idx(0);
+            u32 ib = load_idx(1);
             load_krows(ia, ka);
             for (int ch = 0; ch < nch; ch += 2) {
                 ia = load_idx(ch + 2);
@@ -342,24 +413,23 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        float inv_l[HOME];
+        float inv_l[HC];
 #pragma unroll
-        for (int i = 0; i < HOME; ++i) inv_l[i] = l[i] > 0.f ? 1.0f / l[i] : 0.f;
+        for (int j = 0; j < HC; ++j) inv_l[j] = l[j] > 0.f ? 1.0f / l[j] : 0.f;
 
         v4i oacc[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) oacc[b] = v4i{0, 0, 0, 0};
         auto pass2_chunk = [&](int ch, const v4i (&kop)[4], const u32 (&raw)[16]) {
-            float s[4][4];
+            float s[4][HC];
             int si[4][4];
             logits(ch, kop, s, si);
-            int pq[4][4];
+            int pq[4][HC];
 #pragma unroll
             for (int t = 0; t < 4; ++t)
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    pq[t][i] = i < HOME ? quantize_prob(s[t][i], m[i < HOME ? i : 0], inv_l[i < HOME ? i : 0], p_scale, p_mode) : 0;
-            const v4i pop = p_to_a_layout<HOME>(tiles[wave], pq, lane, home);
+                for (int j = 0; j < HC; ++j) pq[t][j] = quantize_prob(s[t][j], m[j], inv_l[j], p_scale, p_mode);
+            const v4i pop = p_to_a_layout<HC>(tiles[wave], pq, lane, chome, row0);
             if (DBG && M.p_out) {
                 const int slot0 = start + 64 * ch + 16 * g;
                 if (c < V && slot0 < aend)
@@ -378,8 +448,8 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
         {
             v4i ka[4], kb[4];
             u32 va[16], vb[16];
-            int ia = load_idx(0);
-            int ib = load_idx(1);
+            u32 ia = load_idx(0);
+            u32 ib = load_idx(1);
             load_krows(ia, ka);
             load_vrows(ia, va);
             for (int ch = 0; ch < nch_w; ch += 2) {
@@ -395,6 +465,13 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
+        // 1 / row sum of output rows 4g + i, from the lanes that hold those rows
+        float row_inv_l[HOME];
+#pragma unroll
+        for (int i = 0; i < HOME; ++i) {
+            const int src_group = V == 8 ? (g | (i & 2)) : ((4 * g + i) & 3);
+            row_inv_l[i] = __shfl(inv_l[V == 8 ? (i & 1) : 0], 16 * src_group + c, 64);
+        }
         // rows 4g+i, columns 4c + b
         if (home) {
 #pragma unroll
@@ -403,7 +480,7 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
                 if (DBG && M.o_int)
                     *reinterpret_cast<v4i *>(M.o_int + orow) = v4i{oacc[0][i], oacc[1][i], oacc[2][i], oacc[3][i]};
                 // same float operations as dequant_kernel with/without a row scale
-                const float rs = p_mode == kPModeReference ? 1.f : inv_l[i];
+                const float rs = p_mode == kPModeReference ? 1.f : row_inv_l[i];
                 float4 of;
                 of.x = ((float)oacc[0][i] * M.o_scale) * rs;
                 of.y = ((float)oacc[1][i] * M.o_scale) * rs;

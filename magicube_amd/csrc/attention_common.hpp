@@ -21,7 +21,12 @@
 // 48 + c as (e0 + e1) + (e2 + e3), then the 16 lane sums are added as a
 // pairwise tree over lane order; running sum l = l * exp(m - m') + chunk sum.  Both kernels evaluate these statistics
 // on the same register layout (MFMA C layout: lane (c, g) holds rows 4g+i of
-// slot 16t + c) with the same instructions.
+// slot 16t + c) with the same instructions.  The fused kernel first moves
+// the V real rows out of the MFMA tile's padding lanes (rows are dealt to
+// the four 16-lane groups instead of four rows per group), so it keeps fewer
+// rows per lane; each row is still held by the 16 lanes c of one group with
+// slots 16t + c, and its statistics take the same operations in the same
+// order.
 #pragma once
 
 #include "mc_device.hpp"
@@ -60,9 +65,10 @@ __device__ __forceinline__ float row16_sum(float x) {
 }
 
 // Online statistics of one chunk for the HOME rows held by this lane.
-// s[t][i]: scaled logit of slot 16t + c for row 4g + i (-inf when masked).
-template <int HOME>
-__device__ __forceinline__ void softmax_chunk_update(const float (&s)[4][4], float (&m)[HOME], float (&l)[HOME]) {
+// s[t][i]: scaled logit of slot 16t + c for the lane's row i (-inf when masked).
+template <int HOME, int W>
+__device__ __forceinline__ void softmax_chunk_update(const float (&s)[4][W], float (&m)[HOME], float (&l)[HOME]) {
+    static_assert(HOME <= W, "rows held per lane");
 #pragma unroll
     for (int i = 0; i < HOME; ++i) {
         float cm = fmaxf(fmaxf(s[0][i], s[1][i]), fmaxf(s[2][i], s[3][i]));
@@ -90,17 +96,20 @@ __device__ __forceinline__ int quantize_prob(float s, float m, float inv_l, floa
 }
 
 
-// Moves P from the MFMA C layout (lane (c, g): rows 4g+i of slot 16t + c) to
-// the MFMA A layout (lane (r, g): row r, slots 16g..16g+15) through a per-wave
-// 16 x 64 byte LDS tile laid out [row][slot].
-template <int HOME>
-__device__ __forceinline__ v4i p_to_a_layout(unsigned char *tile, const int (&pq)[4][4], int lane, bool home) {
+// Moves P from the lanes' row layout (lane (c, g): rows row0 + i of slot
+// 16t + c; row0 = 4g in the MFMA C layout) to the MFMA A layout (lane (r, g):
+// row r, slots 16g..16g+15) through a per-wave 16 x 64 byte LDS tile laid
+// out [row][slot].
+template <int HOME, int W>
+__device__ __forceinline__ v4i p_to_a_layout(unsigned char *tile, const int (&pq)[4][W], int lane, bool home,
+                                             int row0) {
+    static_assert(HOME <= W, "rows held per lane");
     const int c = lane & 15, g = lane >> 4;
     if (home) {
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int i = 0; i < HOME; ++i) tile[(4 * g + i) * 64 + 16 * t + c] = (unsigned char)pq[t][i];
+            for (int i = 0; i < HOME; ++i) tile[(row0 + i) * 64 + 16 * t + c] = (unsigned char)pq[t][i];
     }
     // The tile is private to the wave and LDS operations of one wave complete
     // in order, so wavefront-scope fences (no waits) are enough: they keep

@@ -64,6 +64,25 @@ def test_fused_equals_plain(qk_bits, p_mode, V):
         assert np.array_equal(po.view(np.uint32), fo.view(np.uint32)), "dequantized O differs"
 
 
+@pytest.mark.parametrize("p_mode", [attention.P_ROW_SCALED, attention.P_REFERENCE])
+@pytest.mark.parametrize("qk_bits", [8, 4])
+@pytest.mark.parametrize("V", [8, 4, 2])
+def test_production_kernel_equals_plain(qk_bits, p_mode, V):
+    """The fused kernel without debug outputs (the one callers run) gives the
+    plain pipeline's fp32 O bit for bit."""
+    mats, host = make_mats([128, 704, 1024], V=V, sparsity=0.8, seed=3 * V + qk_bits + p_mode)
+    p_scale = 127.0 if p_mode == attention.P_ROW_SCALED else 32.0
+    b = attention.AttentionBatch(mats, qk_bits=qk_bits, V=V, p_mode=p_mode, p_scale=p_scale)
+    print(f"production fused vs plain V={V} qk_bits={qk_bits} p_mode={p_mode}", flush=True)
+    b.forward_plain()
+    torch.cuda.synchronize()
+    plain = [d.o.cpu().numpy().copy() for d in b.mats]
+    b.forward()
+    torch.cuda.synchronize()
+    for d, po in zip(b.mats, plain):
+        assert np.array_equal(po.view(np.uint32), d.o.cpu().numpy().view(np.uint32))
+
+
 @pytest.mark.parametrize("qk_bits", [8, 4])
 def test_against_numpy_restatement(qk_bits):
     V = 8
