@@ -210,14 +210,15 @@ __device__ __forceinline__ int const_load(const int *p) {
 // DBG instantiations also write the optional int32 S, int8 P and int32 P V
 // outputs; the production ones carry no stores inside the chunk loops (a
 // conditional store there would make every load wait drain the queue).
+// Four waves per SIMD: the register budget is capped at 128 (a few
+// spills), 7 % faster than three waves with the uncapped allocation.
 template <int V, int QKB, int WPB, bool DBG>
-__global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *mats, const int2 *items, int n_items,
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) void attention_kernel(const AttnMatrix *mats, const int2 *items, int n_items,
                                                              float p_scale, int p_mode) {
     constexpr int HOME = V < 4 ? V : 4;
     constexpr int QROW = kHeadDim * QKB / 8;          // bytes per Q/K row
-    constexpr int KCB = QROW / 4;                     // bytes of a row per lane in a K gather
+    constexpr int KCB = QROW / 4;                     // bytes of a K row per lane
     __shared__ __attribute__((aligned(16))) unsigned char tiles[WPB][16 * 64];
-    __shared__ __attribute__((aligned(16))) unsigned char ktiles[WPB][64 * QROW];
     // the wave id in an SGPR keeps work-item indices, descriptors and buffer
     // resources wave-uniform (no per-lane waterfall around each buffer load)
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -288,51 +289,16 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
             const u32 v = buf_load_b32(rc, slot < end ? (u32)slot * 4u : kRowOob);
             return slot < end ? v * (u32)kHeadDim : kRowOob;
         };
-        // K rows of slots 16t + lane/4, bytes [KCB * (lane % 4), +KCB): four
-        // consecutive lanes read one row, so a gather touches 16 rows per
-        // instruction once each (the MFMA layout would spread each row over
-        // lanes 16 apart and cost four L1 accesses per row segment)
-        auto load_krows = [&](u32 my_off, v4i (&kraw)[4]) {
+        // K rows of slots 16t + c, bytes [KCB * g, +KCB): the MFMA B operand
+        // directly (gathering four lanes per row and moving the rows to this
+        // layout through LDS makes fewer L1 accesses but measured slower)
+        auto load_krows = [&](u32 my_off, v4i (&kop)[4]) {
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                u32 row = (u32)__shfl((int)my_off, 16 * t + (lane >> 2), 64);
+                u32 row = (u32)__shfl((int)my_off, 16 * t + c, 64);
                 if constexpr (QKB == 4) row = (row >> 1) | (row & kRowOob);   // 32-byte rows, marker kept
-                kraw[t] = load_k16<QKB>(rk, row + (u32)(KCB * (lane & 3)));
+                kop[t] = load_k16<QKB>(rk, row + (u32)(KCB * g));
             }
-        };
-        // to the MFMA B layout (lane (c, g): row 16t + c, bytes [KCB * g, +KCB))
-        // through the wave's 64-row K tile; the KCB-byte pieces of a row are
-        // XOR-swizzled by row so that neither side has bank conflicts
-        auto k_to_b_layout = [&](const v4i (&kraw)[4], v4i (&kop)[4]) {
-            unsigned char *kt = ktiles[wave];
-            auto swz = [](int r) { return QKB == 8 ? (r >> 2) & 3 : (r >> 3) & 1; };
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int r = 16 * t + (lane >> 2);
-                unsigned char *dst = kt + r * QROW + KCB * ((lane & 3) ^ swz(r));
-                if constexpr (QKB == 8) *reinterpret_cast<v4i *>(dst) = kraw[t];
-                else *reinterpret_cast<v2i *>(dst) = v2i{kraw[t][0], kraw[t][1]};
-            }
-            // private to the wave, and one wave's LDS operations complete in
-            // order: wavefront-scope fences only keep the compiler from
-            // reordering the reads and writes around each other
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int r = 16 * t + c;
-                const unsigned char *src = kt + r * QROW + KCB * (g ^ swz(r));
-                if constexpr (QKB == 8) {
-                    kop[t] = *reinterpret_cast<const v4i *>(src);
-                } else {
-                    const v2i x = *reinterpret_cast<const v2i *>(src);
-                    kop[t] = v4i{x[0], x[1], 0, 0};
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         };
         // V rows of slots 16g + p (p = 0..15), dword c (columns 4c..4c+3)
         auto load_vrows = [&](u32 my_off, u32 (&raw)[16]) {
@@ -359,9 +325,7 @@ __global__ __launch_bounds__(64 * WPB) void attention_kernel(const AttnMatrix *m
         };
         // logits of one chunk, scaled, in this lane's rows; masked -> -inf.
         // si: the raw C tile (rows 4g + i), kept for the debug outputs only
-        auto logits = [&](int ch, const v4i (&kraw)[4], float (&s)[4][HC], int (&si)[4][4]) {
-            v4i kop[4];
-            k_to_b_layout(kraw, kop);
+        auto logits = [&](int ch, const v4i (&kop)[4], float (&s)[4][HC], int (&si)[4][4]) {
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 v4i kk = kop[t];
