@@ -169,6 +169,9 @@ __global__ __launch_bounds__(64 * WPB) void softmax_requant_kernel(int m_vec, co
     float inv_l[HOME];
 #pragma unroll
     for (int i = 0; i < HOME; ++i) inv_l[i] = l[i] > 0.f ? 1.0f / l[i] : 0.f;
+    float mq[HOME];
+#pragma unroll
+    for (int i = 0; i < HOME; ++i) mq[i] = finite_row_max(m[i]);
     if (row_inv_l && home && c == 0) {
 #pragma unroll
         for (int i = 0; i < HOME; ++i) row_inv_l[(size_t)i_row * V + 4 * g + i] = inv_l[i];
@@ -184,7 +187,7 @@ __global__ __launch_bounds__(64 * WPB) void softmax_requant_kernel(int m_vec, co
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                pq[t][i] = i < HOME ? quantize_prob(s[t][i], m[i < HOME ? i : 0], inv_l[i < HOME ? i : 0], p_scale, p_mode) : 0;
+                pq[t][i] = i < HOME ? quantize_prob(s[t][i], mq[i < HOME ? i : 0], inv_l[i < HOME ? i : 0], p_scale, p_mode) : 0;
         const v4i a = p_to_a_layout<HOME>(tiles[wave], pq, lane, home, 4 * g);
         const int slot0 = start + 64 * ch + 16 * g;        // first slot of this lane's 16-slot group
         if (c < V && slot0 < aend)
@@ -380,6 +383,9 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
         float inv_l[HC];
 #pragma unroll
         for (int j = 0; j < HC; ++j) inv_l[j] = l[j] > 0.f ? 1.0f / l[j] : 0.f;
+        float mq[HC];
+#pragma unroll
+        for (int j = 0; j < HC; ++j) mq[j] = finite_row_max(m[j]);
 
         v4i oacc[4];
 #pragma unroll
@@ -392,7 +398,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
 #pragma unroll
             for (int t = 0; t < 4; ++t)
 #pragma unroll
-                for (int j = 0; j < HC; ++j) pq[t][j] = quantize_prob(s[t][j], m[j], inv_l[j], p_scale, p_mode);
+                for (int j = 0; j < HC; ++j) pq[t][j] = quantize_prob(s[t][j], mq[j], inv_l[j], p_scale, p_mode);
             const v4i pop = p_to_a_layout<HC>(tiles[wave], pq, lane, chome, row0);
             if (DBG && M.p_out) {
                 const int slot0 = start + 64 * ch + 16 * g;

@@ -74,9 +74,12 @@ __device__ __forceinline__ void softmax_chunk_update(const float (&s)[4][W], flo
         float cm = fmaxf(fmaxf(s[0][i], s[1][i]), fmaxf(s[2][i], s[3][i]));
         cm = row16_max(cm);
         const float mn = fmaxf(m[i], cm);        // -inf only while every slot so far is masked
+        // masked slots give exp(-inf) = 0 exactly; the offset is 0 rather
+        // than -inf while the row has no slot yet (-inf - -inf would be NaN)
+        const float mz = mn == -INFINITY ? 0.f : mn;
         float e[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) e[t] = s[t][i] == -INFINITY ? 0.f : __expf(s[t][i] - mn);
+        for (int t = 0; t < 4; ++t) e[t] = __expf(s[t][i] - mz);
         const float csum = row16_sum((e[0] + e[1]) + (e[2] + e[3]));
         const float scale_old = (m[i] == -INFINITY) ? 0.f : __expf(m[i] - mn);
         l[i] = l[i] * scale_old + csum;
@@ -87,9 +90,12 @@ __device__ __forceinline__ void softmax_chunk_update(const float (&s)[4][W], flo
 constexpr int kPModeReference = 0;
 constexpr int kPModeRowScaled = 1;
 
-// Quantized probability of one logit given the final row statistics.
+// Row max as quantize_prob takes it: 0 for a row without slots.
+__device__ __forceinline__ float finite_row_max(float m) { return m == -INFINITY ? 0.f : m; }
+
+// Quantized probability of one logit given the final row statistics
+// (m from finite_row_max); a masked slot (s = -inf) gives exp(-inf) = 0.
 __device__ __forceinline__ int quantize_prob(float s, float m, float inv_l, float p_scale, int p_mode) {
-    if (s == -INFINITY) return 0;
     const float e = __expf(s - m);
     if (p_mode == kPModeReference) return (int)fminf(e * inv_l * p_scale, 127.f);   // truncation (p >= 0)
     return (int)rintf(fminf(e * p_scale, 127.f));
