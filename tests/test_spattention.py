@@ -94,3 +94,19 @@ def test_make_resident_rejects_unaligned_fp16_outputs():
     ok = torch.zeros((L, 128), dtype=torch.float16)[:, 64:]
     d = attention.make_resident(m, (1.0, 1.0, 1.0), 8, 8, 32.0, "cpu", out=ok)
     assert d.o.data_ptr() == ok.data_ptr()
+
+
+def test_make_resident_rejects_too_long_sequences(monkeypatch):
+    import torch
+    from magicube_amd import attention
+    L = 64
+    cols, _, pairs, order, _ = spattention.static_random_mask_aligned(L // 8, L, 0.9, 16, seed=3, device="cpu")
+    mask = spattention.mask_from_reference(order, pairs, cols, L, 8)
+    q = torch.zeros((L, 64), dtype=torch.float16)
+    m = attention.AttentionMatrix(q, q, q, mask)
+    assert attention.MAX_SEQ_LEN == 1 << 24
+    monkeypatch.setattr(attention, "MAX_SEQ_LEN", L - 8)
+    with pytest.raises(ValueError, match="sequence length"):
+        attention.make_resident(m, (1.0, 1.0, 1.0), 8, 8, 32.0, "cpu")
+    monkeypatch.setattr(attention, "MAX_SEQ_LEN", L)
+    attention.make_resident(m, (1.0, 1.0, 1.0), 8, 8, 32.0, "cpu")
