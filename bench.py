@@ -499,7 +499,7 @@ def encoder_bench(reps=10, warmup=3, seed=0):
     return {"workload": "4-layer encoder, seq 4096, embed 256, 4 heads, MLP 1024, sparsity 0.9, V=8, fp16, "
                         "random tokens and weights, inference forward",
             "a100_source": "plot/examples/magicube_n2n.txt, plot/examples/pytorch_n2n.txt",
-            "note": "rhs_pre 4 keeps V int8 (the reference quantizes V to 4 bits)", "cases": cases}
+            "note": "rhs_pre 4: int4 Q and K, V quantized to the 4-bit range as the reference does", "cases": cases}
 
 
 def dense_attention_fp64_torch(q, k, v, mask, V):

@@ -171,7 +171,10 @@ mc_status mc_sddmm_run(int bits, mc_signedness sign, int layout, int m_vec, int 
  * (sample, head) matrix of sequence length L and head dim 64, mask rows
  * aligned to 16 slots:
  *   Qq, Kq, Vq = trunc(clamp(x * scale))          (quantization_kernel.cu:46-74)
- *                or rint(clamp(x * scale)) with MC_ROUND_NEAREST
+ *                or rint(clamp(x * scale)) with MC_ROUND_NEAREST; the clamp
+ *                range is that of the output width, or [-8, 7] for int8
+ *                output with MC_QUANT_RANGE4 (4-bit values, e.g. V of the
+ *                reference's rhs_pre 4 layer, stored one per byte)
  *   S = Qq Kq^T at the mask (int32, exact)
  *   s = float(S) * alpha     alpha = 1 / (scale_q * scale_k * sqrt(64))
  *   P, O by the requantization mode:
@@ -179,13 +182,19 @@ mc_status mc_sddmm_run(int bits, mc_signedness sign, int layout, int m_vec, int 
  *                     O = float(P Vq) * o_scale           (q_softmax_kernel.cu:132-147)
  *     MC_P_ROW_SCALED P = rint(min(exp(s - max) * p_scale, 127)),
  *                     O = (float(P Vq) * o_scale) * (1 / sum)
+ *     MC_P_REFERENCE_4BIT  as MC_P_REFERENCE with P clamped at 7: the
+ *                     reference's 4-bit P (q_softmax_kernel.cu:84-118, lhs_pre 4),
+ *                     held one value per byte
  * The softmax statistics are accumulated over 64-slot chunks in a fixed
  * order, shared by every kernel below, so the one-op-per-launch path and the
  * fused batched path produce identical int32 S, int8 P and int32 P Vq. */
 #define MC_P_REFERENCE 0
 #define MC_P_ROW_SCALED 1
+#define MC_P_REFERENCE_4BIT 2
 #define MC_ROUND_TRUNC 0
 #define MC_ROUND_NEAREST 1
+/* flag, or-ed into the rounding argument of mc_attn_quantize_batched */
+#define MC_QUANT_RANGE4 2
 
 /* One quantization job: count fp16 values -> int8 bytes or packed int4.
  * row_len == 0: x holds the count values contiguously.  Otherwise x holds

@@ -35,8 +35,10 @@ HEAD_DIM = 64
 MAX_SEQ_LEN = 1 << 24
 P_REFERENCE = 0
 P_ROW_SCALED = 1
+P_REFERENCE_4BIT = 2        # P_REFERENCE with P clamped at 7 (4-bit P)
 ROUND_TRUNC = 0
 ROUND_NEAREST = 1
+QUANT_RANGE4 = 2            # or-ed into a rounding mode: 4-bit values stored as int8
 SIGNED = _lib.MC_SIGNED
 
 
@@ -304,13 +306,18 @@ class AttentionBatch:
     def __init__(self, mats: list[AttentionMatrix], qk_bits: int = 8, V: int = 8, p_mode: int = P_ROW_SCALED,
                  p_scale: float = 127.0, scales: list[tuple] | None = None, debug: bool = False,
                  rounding: int | None = None, device="cuda", resident: list[_Dev] | None = None,
-                 outputs: list[torch.Tensor] | None = None):
+                 outputs: list[torch.Tensor] | None = None, v_range4: bool = False):
+        """v_range4: V is quantized to the 4-bit range [-8, 7] (kept one value
+        per byte), as the reference's rhs_pre 4 layer quantizes V."""
         if qk_bits not in (8, 4):
             raise ValueError("qk_bits must be 8 or 4")
+        if p_mode not in (P_REFERENCE, P_ROW_SCALED, P_REFERENCE_4BIT):
+            raise ValueError("unknown p_mode")
         self.qk_bits, self.V, self.p_mode, self.p_scale = qk_bits, V, p_mode, float(p_scale)
         self.debug = bool(debug)
         # the reference truncates (quantization_kernel.cu:65-69); the row-scaled layer rounds
-        self.rounding = rounding if rounding is not None else (ROUND_TRUNC if p_mode == P_REFERENCE else ROUND_NEAREST)
+        self.rounding = rounding if rounding is not None else (ROUND_NEAREST if p_mode == P_ROW_SCALED else ROUND_TRUNC)
+        self.v_rounding = self.rounding | (QUANT_RANGE4 if v_range4 else 0)
         self.device = torch.device(device)
         if resident is not None:
             if any(d.qk_bits != qk_bits for d in resident):
@@ -365,7 +372,7 @@ class AttentionBatch:
         n = len(self.mats)
         _lib.check(lib.mc_attn_quantize_batched(_dptr(self.qk_jobs), 2 * n, self.qk_bits, self.rounding,
                                                 self.max_count, st), "mc_attn_quantize_batched")
-        _lib.check(lib.mc_attn_quantize_batched(_dptr(self.v_jobs), n, 8, self.rounding, self.max_count, st),
+        _lib.check(lib.mc_attn_quantize_batched(_dptr(self.v_jobs), n, 8, self.v_rounding, self.max_count, st),
                    "mc_attn_quantize_batched")
 
     def attend(self, stream=None, grid_waves: int = DEFAULT_GRID_WAVES) -> None:
@@ -410,11 +417,11 @@ class AttentionBatch:
             # fp32 result of the pipeline when the matrix's output is fp16
             w["o32"] = (d.o if d.o.dtype == torch.float32 else
                         torch.empty((d.L, HEAD_DIM), dtype=torch.float32, device=self.device))
-            for name, (x, s, y, b) in {"jq": (d.q16, d.scales[0], d.qq, self.qk_bits),
-                                       "jk": (d.k16, d.scales[1], d.kq, self.qk_bits),
-                                       "jv": (d.v16, d.scales[2], d.vq, 8)}.items():
+            for name, (x, s, y, b, rnd) in {"jq": (d.q16, d.scales[0], d.qq, self.qk_bits, self.rounding),
+                                            "jk": (d.k16, d.scales[1], d.kq, self.qk_bits, self.rounding),
+                                            "jv": (d.v16, d.scales[2], d.vq, 8, self.v_rounding)}.items():
                 w[name] = (_struct_tensor([_QuantJobC(x.data_ptr(), x.numel(), s, y.data_ptr(), *_rows(x))],
-                                          self.device), b)
+                                          self.device), b, rnd)
         return w
 
     def forward_plain(self, stream=None) -> list[torch.Tensor]:
@@ -424,9 +431,8 @@ class AttentionBatch:
         for d in self.mats:
             w = self._plain_buffers(d, stream)
             for name in ("jq", "jk", "jv"):
-                job, b = w[name]
-                _lib.check(lib.mc_attn_quantize_batched(_dptr(job), 1, b, self.rounding, d.L * HEAD_DIM, st),
-                           "quantize")
+                job, b, rnd = w[name]
+                _lib.check(lib.mc_attn_quantize_batched(_dptr(job), 1, b, rnd, d.L * HEAD_DIM, st), "quantize")
             if d.m_vec == 0:
                 continue
             _lib.check(lib.mc_sddmm_run(self.qk_bits, SIGNED, 1, d.m_vec, HEAD_DIM, d.L, _dptr(d.row_indices),
@@ -552,7 +558,7 @@ class BlockBatch:
             raise ValueError("a block batch needs blocks of one width and V")
         dev = torch.device(device)
         self.qk_bits, self.V, self.p_mode, self.p_scale = blocks[0].qk_bits, blocks[0].V, p_mode, float(p_scale)
-        self.rounding = rounding if rounding is not None else (ROUND_TRUNC if p_mode == P_REFERENCE else ROUND_NEAREST)
+        self.rounding = rounding if rounding is not None else (ROUND_NEAREST if p_mode == P_ROW_SCALED else ROUND_TRUNC)
         self.blocks = blocks
         self.n_mats = sum(b.n for b in blocks)
         # raw bytes per block: np.concatenate would repack the padded C

@@ -48,7 +48,9 @@ static_assert(sizeof(QuantJob) == sizeof(mc_quant_job), "job layout");
 template <int BITS>
 __global__ __launch_bounds__(256) void quantize_kernel(const QuantJob *jobs, int njobs, int rounding) {
     const QuantJob jb = jobs[blockIdx.y];
-    const float lo = BITS == 8 ? -128.f : -8.f, hi = BITS == 8 ? 127.f : 7.f;
+    // MC_QUANT_RANGE4 (bit 1 of rounding): 4-bit values in int8 storage
+    const bool range4 = BITS == 4 || (rounding & 2) != 0;
+    const float lo = range4 ? -8.f : -128.f, hi = range4 ? 7.f : 127.f;
     // each thread converts 8 consecutive values
     const long long groups = (jb.count + 7) / 8;
     for (long long gi = (long long)blockIdx.x * 256 + threadIdx.x; gi < groups; gi += (long long)gridDim.x * 256) {
@@ -76,7 +78,7 @@ __global__ __launch_bounds__(256) void quantize_kernel(const QuantJob *jobs, int
         for (int e = 0; e < 8; ++e) {
             float f = x[e] * jb.scale;
             f = fminf(fmaxf(f, lo), hi);
-            q[e] = rounding == 0 ? (int)f : (int)rintf(f);
+            q[e] = (rounding & 1) == 0 ? (int)f : (int)rintf(f);
         }
         if constexpr (BITS == 8) {
             unsigned char *o = static_cast<unsigned char *>(jb.out) + base;
@@ -450,7 +452,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
                 if (DBG && M.o_int)
                     *reinterpret_cast<v4i *>(M.o_int + orow) = v4i{oacc[0][i], oacc[1][i], oacc[2][i], oacc[3][i]};
                 // same float operations as dequant_kernel with/without a row scale
-                const float rs = p_mode == kPModeReference ? 1.f : row_inv_l[i];
+                const float rs = p_mode == kPModeRowScaled ? row_inv_l[i] : 1.f;
                 float4 of;
                 of.x = ((float)oacc[0][i] * M.o_scale) * rs;
                 of.y = ((float)oacc[1][i] * M.o_scale) * rs;
@@ -493,7 +495,8 @@ extern "C" {
 mc_status mc_attn_quantize_batched(const mc_quant_job *jobs, int njobs, int bits, int rounding,
                                    long long max_count, hipStream_t stream) {
     if (bits != 8 && bits != 4) return MC_EUNSUPPORTED;
-    if (rounding != MC_ROUND_TRUNC && rounding != MC_ROUND_NEAREST) return MC_EINVAL_ARG;
+    if ((rounding & ~MC_QUANT_RANGE4) != MC_ROUND_TRUNC && (rounding & ~MC_QUANT_RANGE4) != MC_ROUND_NEAREST)
+        return MC_EINVAL_ARG;
     if (njobs < 0 || max_count < 0) return MC_EINVAL_ARG;
     if (njobs == 0 || max_count == 0) return MC_OK;
     if (!jobs || njobs > 65535) return MC_EINVAL_ARG;
@@ -527,7 +530,7 @@ mc_status mc_attn_dequant(const int *in, int rows, int cols, float scale, const 
 mc_status mc_attn_softmax_requant(int m_vec, int vec_length, const int *row_indices, const int *row_offsets,
                                   const int *s_vals, float alpha, float p_scale, int p_mode, void *p_out,
                                   float *row_inv_l, hipStream_t stream) {
-    if (p_mode != MC_P_REFERENCE && p_mode != MC_P_ROW_SCALED) return MC_EINVAL_ARG;
+    if (p_mode != MC_P_REFERENCE && p_mode != MC_P_ROW_SCALED && p_mode != MC_P_REFERENCE_4BIT) return MC_EINVAL_ARG;
     if (!mc::valid_v(vec_length)) return MC_EINVAL_V;
     if (m_vec < 0) return MC_EINVAL_SHAPE;
     if (m_vec == 0) return MC_OK;
@@ -546,7 +549,7 @@ mc_status mc_attn_softmax_requant(int m_vec, int vec_length, const int *row_indi
 mc_status mc_attn_forward(const mc_attn_matrix *matrices, const int *items, int n_items, int qk_bits, int vec_length,
                           float p_scale, int p_mode, int grid_waves, int debug, hipStream_t stream) {
     if (qk_bits != 8 && qk_bits != 4) return MC_EUNSUPPORTED;
-    if (p_mode != MC_P_REFERENCE && p_mode != MC_P_ROW_SCALED) return MC_EINVAL_ARG;
+    if (p_mode != MC_P_REFERENCE && p_mode != MC_P_ROW_SCALED && p_mode != MC_P_REFERENCE_4BIT) return MC_EINVAL_ARG;
     if (!mc::valid_v(vec_length)) return MC_EINVAL_V;
     if (n_items < 0 || grid_waves < 0) return MC_EINVAL_ARG;
     if (n_items == 0) return MC_OK;

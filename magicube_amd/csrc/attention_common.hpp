@@ -89,6 +89,7 @@ __device__ __forceinline__ void softmax_chunk_update(const float (&s)[4][W], flo
 
 constexpr int kPModeReference = 0;
 constexpr int kPModeRowScaled = 1;
+constexpr int kPModeReference4 = 2;   // reference requantization into 4-bit P
 
 // Row max as quantize_prob takes it: 0 for a row without slots.
 __device__ __forceinline__ float finite_row_max(float m) { return m == -INFINITY ? 0.f : m; }
@@ -97,7 +98,8 @@ __device__ __forceinline__ float finite_row_max(float m) { return m == -INFINITY
 // (m from finite_row_max); a masked slot (s = -inf) gives exp(-inf) = 0.
 __device__ __forceinline__ int quantize_prob(float s, float m, float inv_l, float p_scale, int p_mode) {
     const float e = __expf(s - m);
-    if (p_mode == kPModeReference) return (int)fminf(e * inv_l * p_scale, 127.f);   // truncation (p >= 0)
+    if (p_mode != kPModeRowScaled)   // truncation (p >= 0)
+        return (int)fminf(e * inv_l * p_scale, p_mode == kPModeReference4 ? 7.f : 127.f);
     return (int)rintf(fminf(e * p_scale, 127.f));
 }
 

@@ -20,12 +20,21 @@ over the shared mask.  The module keeps its staging buffers, quantized
 operands, outputs and launch descriptors per (shape, mask), so later calls
 only launch kernels and can be captured in a HIP graph.
 
-Precisions: ``lhs_pre`` 8 (int8 P) with ``rhs_pre`` 8 (int8 Q, K, V) or 4
-(int4 Q, K).  V stays int8 in both: the reference quantizes V to ``rhs_pre``
-bits too (spattention.py:191), so with ``rhs_pre`` 4 this layer's V is more
-precise than the reference's.  ``need_weights=True`` and
-``key_padding_mask`` are rejected; the reference's encoder uses neither
-(end_to_end.py:70-73).
+Precisions (``lhs_pre`` = P bits, ``rhs_pre`` = Q/K/V bits), the reference's
+four pairs (spattention.py:194-215):
+
+* 8, 8: int8 Q, K, V and P;
+* 16, 8: the same integers as 8, 8.  P = trunc(min(... * 32, 32767)) never
+  exceeds 32, so the reference's 16-bit P holds exactly the int8 P values and
+  P V is the same exact int32 product (its bspmm_16b8b splits P into byte
+  planes);
+* 8, 4: int4 Q and K; V quantized to the 4-bit range [-8, 7], as the
+  reference quantizes it (spattention.py:191), held one value per byte;
+* 4, 4: as 8, 4 with P clamped at 7 (the reference's 4-bit P,
+  q_softmax_kernel.cu:84-118).
+
+``need_weights=True`` and ``key_padding_mask`` are rejected; the reference's
+encoder uses neither (end_to_end.py:70-73).
 
 ``SparseTransformerEncoder`` / ``TransformerEncoder`` restate the encoder
 benchmark models (end_to_end.py:47-183): embedding, sinusoidal positions,
@@ -42,7 +51,8 @@ import torch.nn.functional as F
 from torch import nn
 
 from . import formats
-from .attention import HEAD_DIM, P_REFERENCE, ROUND_TRUNC, AttentionBatch, AttentionMatrix, ResidentMask
+from .attention import (HEAD_DIM, P_REFERENCE, P_REFERENCE_4BIT, ROUND_TRUNC, AttentionBatch, AttentionMatrix,
+                        ResidentMask)
 
 SCALE_QKV = 36.0    # spattention.py:51
 SCALE_SFMX = 32.0   # spattention.py:53
@@ -101,7 +111,7 @@ class _Plan:
     ([L][batch][E], the output projection's input), so no permute copies run
     between the projections and the attention kernels."""
 
-    def __init__(self, L: int, bsz: int, heads: int, mask: ResidentMask, rhs_pre: int, device):
+    def __init__(self, L: int, bsz: int, heads: int, mask: ResidentMask, lhs_pre: int, rhs_pre: int, device):
         E = heads * HEAD_DIM
         self.qkv = torch.empty((L, bsz, 3 * E), dtype=torch.float16, device=device)
         self.o = torch.empty((L, bsz, E), dtype=torch.float16, device=device)
@@ -113,9 +123,10 @@ class _Plan:
                 mats.append(AttentionMatrix(q5[:, b, 0, h], q5[:, b, 1, h], q5[:, b, 2, h], mask))
                 outs.append(o4[:, b, h])
         n_mat = bsz * heads
-        self.batch = AttentionBatch(mats, qk_bits=rhs_pre, V=mask.V, p_mode=P_REFERENCE, p_scale=SCALE_SFMX,
+        self.batch = AttentionBatch(mats, qk_bits=rhs_pre, V=mask.V,
+                                    p_mode=P_REFERENCE_4BIT if lhs_pre == 4 else P_REFERENCE, p_scale=SCALE_SFMX,
                                     scales=[(SCALE_QKV, SCALE_QKV, SCALE_QKV)] * n_mat, rounding=ROUND_TRUNC,
-                                    device=device, outputs=outs)
+                                    device=device, outputs=outs, v_range4=rhs_pre == 4)
 
 
 class SpMultiheadAttention(nn.MultiheadAttention):
@@ -137,9 +148,10 @@ class SpMultiheadAttention(nn.MultiheadAttention):
             raise NotImplementedError("attention weights are not materialized: call with need_weights=False")
         if key_padding_mask is not None:
             raise NotImplementedError("key_padding_mask is not supported")
-        if lhs_pre != 8 or rhs_pre not in (8, 4):
-            raise NotImplementedError(f"precisions lhs_pre={lhs_pre}, rhs_pre={rhs_pre}: supported are "
-                                      "lhs_pre 8 with rhs_pre 8 or 4")
+        if (lhs_pre, rhs_pre) not in ((8, 8), (16, 8), (8, 4), (4, 4)):
+            raise NotImplementedError(f"precisions lhs_pre={lhs_pre}, rhs_pre={rhs_pre}: the reference's pairs "
+                                      "are (8, 8), (16, 8), (8, 4) and (4, 4)")
+        p_bits = 4 if lhs_pre == 4 else 8       # 16-bit P holds the int8 P values (P <= 32)
         L, bsz, E = query.shape
         H = self.num_heads
         if key.shape != query.shape or value.shape != query.shape:
@@ -147,16 +159,16 @@ class SpMultiheadAttention(nn.MultiheadAttention):
         if mask is None:
             if row_offsets is None or column_indices is None:
                 raise ValueError("a mask (row_offsets, column_indices) is required")
-            key_ = (L, bsz, rhs_pre, vec_length, row_offsets.data_ptr(), column_indices.data_ptr())
+            key_ = (L, bsz, p_bits, rhs_pre, vec_length, row_offsets.data_ptr(), column_indices.data_ptr())
         else:
-            key_ = (L, bsz, rhs_pre, mask.V, id(mask))
+            key_ = (L, bsz, p_bits, rhs_pre, mask.V, id(mask))
         plan = self._plans.get(key_)
         if plan is None:
             if mask is None:
                 ri = row_indices if row_indices is not None else torch.zeros(1, dtype=torch.int32,
                                                                                device=query.device)
                 mask = mask_from_reference(ri, row_offsets, column_indices, L, vec_length)
-            plan = _Plan(L, bsz, H, mask, rhs_pre, query.device)
+            plan = _Plan(L, bsz, H, mask, p_bits, rhs_pre, query.device)
             self._plans[key_] = plan
 
         # in-projection into the plan's [L][batch][3E] buffer, read in place

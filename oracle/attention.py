@@ -8,7 +8,7 @@
   (quantization_kernel.cu:46-74 for 8 bits, :13-43 for 4 bits, truncating,
   or rounding to nearest in the row-scaled layer;
   spattention.py:182-216; q_softmax_kernel.cu:132-147 for the reference
-  requantization) — exact integers for Qq/Kq/Vq and S; FP32 softmax in numpy's
+  requantization, :84-118 for its 4-bit P, p_mode 2) — exact integers for Qq/Kq/Vq and S; FP32 softmax in numpy's
   own summation order (the GPU's fixed order may differ by one rounding, so P
   is compared within one quantization step); P V and the dequantization given
   a P are exact.
@@ -53,15 +53,19 @@ def quantize_ref(x16: np.ndarray, scale: float, bits: int, rounding: int = 0) ->
     return (np.rint(f) if rounding else np.trunc(f)).astype(np.int32)
 
 
-def quantized_attention_ref(q16, k16, v16, csr, scales, qk_bits=8, p_mode=1, p_scale=127.0, rounding=None):
+def quantized_attention_ref(q16, k16, v16, csr, scales, qk_bits=8, p_mode=1, p_scale=127.0, rounding=None,
+                            v_bits=8):
     """Returns dict(qq, kq, vq, S (list of [V, nnz] int64), P (list of [V, nnz] int),
-    o_int [L, 64] int64 computed from this P, o float32)."""
+    o_int [L, 64] int64 computed from this P, o float32).  p_mode 0: reference
+    requantization (P clamped at 127), 1: row-scaled, 2: reference into 4-bit
+    P (clamped at 7).  v_bits: the range V is quantized to (8 or 4)."""
     sq, sk, sv = scales
     if rounding is None:
-        rounding = 0 if p_mode == 0 else 1
+        rounding = 1 if p_mode == 1 else 0
     qq = quantize_ref(q16, sq, qk_bits, rounding)
     kq = quantize_ref(k16, sk, qk_bits, rounding)
-    vq = quantize_ref(v16, sv, 8, rounding)
+    vq = quantize_ref(v16, sv, v_bits, rounding)
+    p_max = np.float32(7.0 if p_mode == 2 else 127.0)
     V = csr.V
     alpha = np.float32(1.0) / (np.float32(sq) * np.float32(sk) * np.float32(math.sqrt(HEAD_DIM)))
     o_scale = np.float32(1.0) / (np.float32(sv) * np.float32(p_scale))
@@ -80,10 +84,10 @@ def quantized_attention_ref(q16, k16, v16, csr, scales, qk_bits=8, p_mode=1, p_s
         e = np.exp(s - m).astype(np.float32)
         l = e.sum(axis=1, dtype=np.float32, keepdims=True)
         inv_l = np.float32(1.0) / l
-        if p_mode == 0:
-            P = np.trunc(np.minimum(e * inv_l * np.float32(p_scale), np.float32(127.0))).astype(np.int64)
+        if p_mode != 1:
+            P = np.trunc(np.minimum(e * inv_l * np.float32(p_scale), p_max)).astype(np.int64)
         else:
-            P = np.rint(np.minimum(e * np.float32(p_scale), np.float32(127.0))).astype(np.int64)
+            P = np.rint(np.minimum(e * np.float32(p_scale), p_max)).astype(np.int64)
         P_rows.append(P)
         oi = P @ vq[cols].astype(np.int64)
         o_int[i * V:(i + 1) * V] = oi

@@ -64,15 +64,16 @@ def test_fused_equals_plain(qk_bits, p_mode, V):
         assert np.array_equal(po.view(np.uint32), fo.view(np.uint32)), "dequantized O differs"
 
 
-@pytest.mark.parametrize("p_mode", [attention.P_ROW_SCALED, attention.P_REFERENCE])
+@pytest.mark.parametrize("p_mode", [attention.P_ROW_SCALED, attention.P_REFERENCE, attention.P_REFERENCE_4BIT])
 @pytest.mark.parametrize("qk_bits", [8, 4])
 @pytest.mark.parametrize("V", [8, 4, 2])
 def test_production_kernel_equals_plain(qk_bits, p_mode, V):
     """The fused kernel without debug outputs (the one callers run) gives the
-    plain pipeline's fp32 O bit for bit."""
+    plain pipeline's fp32 O bit for bit (4-bit P runs with V in the 4-bit range)."""
     mats, host = make_mats([128, 704, 1024], V=V, sparsity=0.8, seed=3 * V + qk_bits + p_mode)
     p_scale = 127.0 if p_mode == attention.P_ROW_SCALED else 32.0
-    b = attention.AttentionBatch(mats, qk_bits=qk_bits, V=V, p_mode=p_mode, p_scale=p_scale)
+    b = attention.AttentionBatch(mats, qk_bits=qk_bits, V=V, p_mode=p_mode, p_scale=p_scale,
+                                 v_range4=p_mode == attention.P_REFERENCE_4BIT)
     print(f"production fused vs plain V={V} qk_bits={qk_bits} p_mode={p_mode}", flush=True)
     b.forward_plain()
     torch.cuda.synchronize()

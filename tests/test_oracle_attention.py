@@ -118,3 +118,25 @@ def test_planted_mistakes_move_far_from_fp64(mistake):
         vv = (-v.astype(np.float32)).astype(np.float16)
     bad = oatt.error_vs_fp64(oatt.quantized_attention_ref(qq, kk, vv, mm, tuple(sc))["o"], ref)
     assert bad["rel_fro"] > 0.2, (mistake, bad)
+
+
+def test_4bit_p_and_v_range_modes():
+    """p_mode 2 is the reference requantization clamped at 7 (a single-column
+    row gives P = min(32, 7) = 7; elsewhere P = min(P_reference, 7) since
+    trunc(min(x, 7)) = min(trunc(min(x, 127)), 7) for x >= 0), and v_bits 4
+    clamps V to [-8, 7] before truncation."""
+    L, V = 32, 8
+    q, k, v = synthetic.attention_inputs(L, 3)
+    cols = np.array([5, 17, 0, 31], np.int32)
+    mask = formats.VectorCSR(4, L, V, np.arange(5, dtype=np.int32), cols)
+    r = oatt.quantized_attention_ref(q, k, v, mask, (30.0, 31.0, 32.0), p_mode=2, p_scale=32.0)
+    assert all((P == 7).all() for P in r["P"])
+    q, k, v, mask = _tiny(L=64, sparsity=0.5, seed=9)
+    sc = (36.0, 36.0, 36.0 * 8)
+    r0 = oatt.quantized_attention_ref(q, k, v, mask, sc, p_mode=0, p_scale=32.0)
+    r2 = oatt.quantized_attention_ref(q, k, v, mask, sc, p_mode=2, p_scale=32.0, v_bits=4)
+    for P0, P2 in zip(r0["P"], r2["P"]):
+        assert np.array_equal(P2, np.minimum(P0, 7))
+    assert np.array_equal(r2["vq"], np.clip(r0["vq"], -8, 7))
+    assert r2["vq"].min() == -8 and r2["vq"].max() == 7       # the clamp is exercised
+    assert any((P0 > 7).any() for P0 in r0["P"])

@@ -12,21 +12,26 @@ from magicube_amd import formats, spattention
 pytestmark = pytest.mark.gpu
 
 
-def _layer(L, bsz, heads, rhs_pre, seed):
+def _layer(L, bsz, heads, rhs_pre, seed, lhs_pre=8):
     torch.manual_seed(seed)
     E = 64 * heads
     mha = spattention.SpMultiheadAttention(embed_dim=E, num_heads=heads).cuda().half().eval()
     align = 32 if rhs_pre == 4 else 16
     cols, _, pairs, order, _ = spattention.static_random_mask_aligned(L // 8, L, 0.9, align, seed=seed)
     x = (torch.randn(L, bsz, E, device="cuda") * 0.5).half()
-    return mha, x, dict(row_indices=order, row_offsets=pairs, column_indices=cols, vec_length=8, rhs_pre=rhs_pre)
+    return mha, x, dict(row_indices=order, row_offsets=pairs, column_indices=cols, vec_length=8, rhs_pre=rhs_pre,
+                        lhs_pre=lhs_pre)
 
 
-@pytest.mark.parametrize("rhs_pre", [8, 4])
-def test_module_equals_plain_pipeline_and_restatement(orc, rhs_pre):
+PAIRS = [(8, 8), (16, 8), (8, 4), (4, 4)]
+
+
+@pytest.mark.parametrize("pair", PAIRS, ids=lambda p: f"L{p[0]}R{p[1]}")
+def test_module_equals_plain_pipeline_and_restatement(orc, pair):
+    lhs_pre, rhs_pre = pair
     L, bsz, heads = 512, 2, 2
-    mha, x, kw = _layer(L, bsz, heads, rhs_pre, seed=7)
-    print(f"spattention L={L} bsz={bsz} heads={heads} rhs_pre={rhs_pre}", flush=True)
+    mha, x, kw = _layer(L, bsz, heads, rhs_pre, seed=7, lhs_pre=lhs_pre)
+    print(f"spattention L={L} bsz={bsz} heads={heads} lhs_pre={lhs_pre} rhs_pre={rhs_pre}", flush=True)
     with torch.no_grad():
         out, _ = mha(x, x, x, need_weights=False, **kw)
         torch.cuda.synchronize()
@@ -50,7 +55,8 @@ def test_module_equals_plain_pipeline_and_restatement(orc, rhs_pre):
     csr = formats.VectorCSR(L // 8, L, 8, ro, ci)
     for i in range(bsz * heads):
         ref = oatt.quantized_attention_ref(stage[0, i], stage[1, i], stage[2, i], csr, (36.0, 36.0, 36.0),
-                                           qk_bits=rhs_pre, p_mode=0, p_scale=32.0, rounding=0)
+                                           qk_bits=rhs_pre, p_mode=2 if lhs_pre == 4 else 0, p_scale=32.0,
+                                           rounding=0, v_bits=rhs_pre)
         # P agrees with the FP32 restatement to within one step (1/32 of a
         # probability) at a few slots; each such slot moves O by at most |v| / 32
         # slot moves O by at most |v| / 32; the fp16 store adds half an fp16 ulp
@@ -59,6 +65,31 @@ def test_module_equals_plain_pipeline_and_restatement(orc, rhs_pre):
         diff = np.abs(got - ref["o"])
         assert diff.max() <= 4 * vmax / 32.0 + 2.0 ** -11 * np.abs(ref["o"]).max() + 1e-6, f"matrix {i}"
         assert np.linalg.norm(got - ref["o"]) <= 1e-2 * np.linalg.norm(ref["o"]), f"matrix {i}"
+
+
+def test_16bit_p_equals_8bit_p():
+    """P <= 32 at the reference's scales, so lhs_pre 16 gives the lhs_pre 8 layer's output bit for bit."""
+    L, bsz, heads = 256, 2, 2
+    mha, x, kw = _layer(L, bsz, heads, 8, seed=13)
+    print("spattention lhs_pre 16 vs 8", flush=True)
+    with torch.no_grad():
+        o8, _ = mha(x, x, x, need_weights=False, **kw)
+        o16, _ = mha(x, x, x, need_weights=False, **dict(kw, lhs_pre=16))
+        torch.cuda.synchronize()
+    assert torch.equal(o8, o16)
+
+
+def test_4bit_p_and_v_ranges():
+    """lhs_pre 4 clamps P at 7; rhs_pre 4 keeps V in [-8, 7]."""
+    L, bsz, heads = 256, 1, 2
+    mha, x, kw = _layer(L, bsz, heads, 4, seed=17, lhs_pre=4)
+    print("spattention lhs_pre 4 rhs_pre 4 ranges", flush=True)
+    with torch.no_grad():
+        mha(x * 8, x * 8, x * 8, need_weights=False, **kw)
+        torch.cuda.synchronize()
+        plan = next(iter(mha._plans.values()))
+        vq = torch.cat([d.vq.view(torch.int8).flatten().cpu() for d in plan.batch.mats])
+    assert int(vq.min()) >= -8 and int(vq.max()) <= 7 and int(vq.min()) == -8 and int(vq.max()) == 7
 
 
 def test_module_output_projection_and_layout():
