@@ -442,7 +442,9 @@ def sddmm_published_bench(steps, warmup, seed=43):
 # 4096, sparsity 0.9, V = 8, fp16 model), mean of 8 runs:
 # (batch, lhs, rhs) -> (ms, plot/examples/magicube_n2n.txt lines); dense from pytorch_n2n.txt.
 PUBLISHED_ENCODER = {(2, 8, 8): (3.731, "19-26"), (2, 8, 4): (3.371, "10-17"),
-                     (8, 8, 8): (13.261, "55-62"), (8, 8, 4): (12.459, "46-53")}
+                     (2, 16, 8): (3.754, "28-35"), (2, 4, 4): (3.338, "1-8"),
+                     (8, 8, 8): (13.261, "55-62"), (8, 8, 4): (12.459, "46-53"),
+                     (8, 16, 8): (13.443, "64-71"), (8, 4, 4): (12.048, "37-44")}
 PUBLISHED_ENCODER_DENSE = {2: (5.947, "1-8"), 8: (24.541, "10-17")}
 
 
@@ -475,7 +477,7 @@ def encoder_bench(reps=10, warmup=3, seed=0):
     with torch.no_grad():
         for bs in (2, 8):
             x = torch.randint(0, 257, (bs, 4096), dtype=torch.int32, device="cuda")
-            for (lp, rp) in ((8, 8), (8, 4)):
+            for (lp, rp) in ((8, 8), (16, 8), (8, 4), (4, 4)):
                 model = spattention.SparseTransformerEncoder(257, 256, 4, 0.9, 4096, 1024, 1024, vec_length=8,
                                                              lhs_pre=lp, rhs_pre=rp, seed=seed).cuda().half().eval()
                 eager = timed(lambda: model(x))
