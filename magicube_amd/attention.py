@@ -600,7 +600,12 @@ class BlockBatch:
 
 def bquantization(x: torch.Tensor, bits: int, scale: float, rounding: int = ROUND_TRUNC,
                   stream=None) -> torch.Tensor:
-    """fp16 [..., L, D] -> int32-packed [..., L, D*bits/32] (trunc(clamp(x*scale)) by default)."""
+    """fp16 [..., L, D] -> int32-packed [..., L, D*bits/32] (trunc(clamp(x*scale)) by default).
+    D * bits must be a multiple of 32 (the reference passes D = 64)."""
+    if bits not in (8, 4):
+        raise ValueError("bits must be 8 or 4")
+    if x.dim() == 0 or (x.shape[-1] * bits) % 32 != 0:
+        raise ValueError(f"the last dimension ({x.shape[-1] if x.dim() else 0}) times bits must be a multiple of 32")
     x = x.contiguous()
     out = torch.empty(x.shape[:-1] + (x.shape[-1] * bits // 32,), dtype=torch.int32, device=x.device)
     job = _struct_tensor([_QuantJobC(x.data_ptr(), x.numel(), float(scale), out.data_ptr())], x.device)

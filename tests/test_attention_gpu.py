@@ -40,6 +40,24 @@ def _debug_outputs(batch):
             for d in batch.mats]
 
 
+@pytest.mark.parametrize("rounding", [attention.ROUND_TRUNC, attention.ROUND_NEAREST])
+def test_quantize_4bit_range_in_int8(rounding):
+    """MC_QUANT_RANGE4: int8 output clamped to [-8, 7], equal to the 4-bit
+    restatement and to the 4-bit packed kernel's nibbles."""
+    rng = np.random.default_rng(5)
+    x = (rng.standard_normal(4104) * 3).astype(np.float16)          # rows of 8 values
+    xt = torch.from_numpy(x).cuda().view(1, -1, 8)
+    got = attention.bquantization(xt, 8, 2.5, rounding=rounding | attention.QUANT_RANGE4)
+    assert got.numel() * 4 == x.size
+    exp = oatt.quantize_ref(x, 2.5, 4, rounding)
+    assert np.array_equal(got.cpu().numpy().view(np.int8).astype(np.int32).ravel(), exp)
+    assert exp.min() == -8 and exp.max() == 7
+    packed = attention.bquantization(xt, 4, 2.5, rounding=rounding).cpu().numpy().view(np.uint32).ravel()
+    assert packed.size * 8 == x.size
+    nib = np.stack([(packed >> (4 * e)) & 15 for e in range(8)], axis=1).ravel().astype(np.int32)
+    assert np.array_equal(np.where(nib > 7, nib - 16, nib), exp)
+
+
 @pytest.mark.parametrize("p_mode", [attention.P_ROW_SCALED, attention.P_REFERENCE])
 @pytest.mark.parametrize("qk_bits", [8, 4])
 @pytest.mark.parametrize("V", [8, 4, 2])

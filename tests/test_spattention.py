@@ -110,3 +110,14 @@ def test_make_resident_rejects_too_long_sequences(monkeypatch):
         attention.make_resident(m, (1.0, 1.0, 1.0), 8, 8, 32.0, "cpu")
     monkeypatch.setattr(attention, "MAX_SEQ_LEN", L)
     attention.make_resident(m, (1.0, 1.0, 1.0), 8, 8, 32.0, "cpu")
+
+
+def test_bquantization_rejects_shapes_it_cannot_pack():
+    import torch
+    from magicube_amd import attention
+    with pytest.raises(ValueError):
+        attention.bquantization(torch.zeros(4099, dtype=torch.float16), 8, 1.0)   # 4099 bytes in 1024 words
+    with pytest.raises(ValueError):
+        attention.bquantization(torch.zeros(2, 12, dtype=torch.float16), 4, 1.0)  # 12 nibbles per row
+    with pytest.raises(ValueError):
+        attention.bquantization(torch.zeros(2, 64, dtype=torch.float16), 16, 1.0)
