@@ -13,7 +13,7 @@ import numpy as np
 import torch
 
 from . import _lib, formats
-from .spmm import _dptr, _stream
+from .spmm import _check_bytes, _dptr, _stream
 
 SIGNED = _lib.MC_SIGNED
 UNSIGNED = _lib.MC_UNSIGNED
@@ -24,6 +24,9 @@ OUT_VECTOR = 1      # [j][v]
 def wmma_sddmm(bits: int, m_vec: int, k: int, n: int, row_indices, row_offsets, col_indices, lhs_matrix,
                rhs_matrix, output_values, vec_length: int, sign: int = SIGNED, layout: int = OUT_REFERENCE,
                stream=None) -> None:
+    if min(m_vec, k, n, vec_length) >= 0:
+        _check_bytes(lhs_matrix, m_vec * vec_length * k * bits // 8, "lhs_matrix")
+        _check_bytes(rhs_matrix, n * k * bits // 8, "rhs_matrix")
     st = _lib.lib().mc_sddmm_run(bits, sign, layout, m_vec, k, n, _dptr(row_indices), _dptr(row_offsets),
                                  _dptr(col_indices), _dptr(lhs_matrix), _dptr(rhs_matrix), _dptr(output_values),
                                  vec_length, _stream(stream))

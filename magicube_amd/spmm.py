@@ -47,9 +47,19 @@ def _stream(stream):
     return ctypes.c_void_p(int(stream))
 
 
+def _check_bytes(t, need: int, what: str) -> None:
+    """A tensor operand must hold at least ``need`` bytes (raw integer
+    pointers are the caller's responsibility, as in the C ABI)."""
+    if isinstance(t, torch.Tensor) and t.numel() * t.element_size() < need:
+        raise ValueError(f"{what} holds {t.numel() * t.element_size()} bytes; the launch reads or writes {need}")
+
+
 def wmma_spmm(lhs_bits: int, rhs_bits: int, m_vec: int, vec_length: int, n: int, k: int, row_indices,
               row_offsets, column_indices, values, rhs_matrix, output_matrix, sign: int = SIGNED,
               config: int | None = None, stream=None) -> None:
+    if min(m_vec, vec_length, n, k) >= 0:
+        _check_bytes(output_matrix, m_vec * vec_length * n * 4, "output_matrix")
+        _check_bytes(rhs_matrix, k * n * rhs_bits // 8, "rhs_matrix")
     lib = _lib.lib()
     args = (m_vec, vec_length, n, k, _dptr(row_indices), _dptr(row_offsets), _dptr(column_indices),
             _dptr(values), _dptr(rhs_matrix), _dptr(output_matrix))

@@ -121,3 +121,18 @@ def test_bquantization_rejects_shapes_it_cannot_pack():
         attention.bquantization(torch.zeros(2, 12, dtype=torch.float16), 4, 1.0)  # 12 nibbles per row
     with pytest.raises(ValueError):
         attention.bquantization(torch.zeros(2, 64, dtype=torch.float16), 16, 1.0)
+
+
+def test_spmm_and_sddmm_mirrors_reject_short_buffers():
+    import torch
+    from magicube_amd import sddmm, spmm
+    z = torch.zeros(16, dtype=torch.int32)
+    with pytest.raises(ValueError, match="output_matrix"):
+        spmm.wmma_spmm(8, 8, 4, 8, 64, 64, z, z, z, z, torch.zeros(64 * 64 // 4, dtype=torch.int32),
+                       torch.zeros(4 * 8 * 64 - 1, dtype=torch.int32))
+    with pytest.raises(ValueError, match="rhs_matrix"):
+        spmm.wmma_spmm(8, 4, 4, 8, 64, 64, z, z, z, z, torch.zeros(64 * 64 // 8 - 1, dtype=torch.int32),
+                       torch.zeros(4 * 8 * 64, dtype=torch.int32))
+    with pytest.raises(ValueError, match="lhs_matrix"):
+        sddmm.wmma_sddmm(8, 4, 64, 64, z, z, z, torch.zeros(4 * 8 * 64 // 4 - 1, dtype=torch.int32),
+                         torch.zeros(64 * 64 // 4, dtype=torch.int32), z, 8)
