@@ -27,6 +27,7 @@ import json
 import os
 import statistics
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -166,15 +167,19 @@ def checksum_ok(C, csr, a, b, K, N, V, seed=0):
     return bool(np.array_equal(got & 0xFFFFFFFF, exp.reshape(-1) & 0xFFFFFFFF))
 
 
-def time_gpu(run_once, steps, warmup, stream, warm_s=0.02):
-    """Median per-step time (ms) of ``steps`` graph-replayed launches.
+def time_gpu(run_once, steps, warmup, stream, warm_s=0.02, detail=False):
+    """Per-step time (ms) of ``steps`` graph-replayed launches.
 
     A few eager launches initialize the work; the ``steps`` launches are then
     captured once into a HIP graph, the graph is replayed untimed until
     ``warmup`` steps and at least ``warm_s`` seconds of back-to-back work
     have run (the GPU reaches its working clocks only under sustained load,
     not between eager launches or host-side setup), and five timed replays
-    follow, each bracketed by HIP events on the capture stream."""
+    follow, each bracketed by torch.cuda.synchronize() and by HIP events on
+    the capture stream.  Returns the median over the five and the graph; with
+    ``detail`` a dict instead: the first timed replay (exactly ``steps``
+    launches) as ``ms``, the median, the host wall time of that replay and
+    the warm-up actually run."""
     import torch
     eager = min(warmup, 3)
     for _ in range(eager):
@@ -187,7 +192,7 @@ def time_gpu(run_once, steps, warmup, stream, warm_s=0.02):
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     reps = 5
-    times = []
+    times, walls = [], []
     # replay() launches on the current stream: keep it the capture stream so
     # the events bracket exactly the replayed launches
     with torch.cuda.stream(stream):
@@ -199,56 +204,138 @@ def time_gpu(run_once, steps, warmup, stream, warm_s=0.02):
             if done % 8 == 0:
                 torch.cuda.synchronize()
         torch.cuda.synchronize()
+        warm_wall = time.perf_counter() - t0
         for _ in range(reps):
             torch.cuda.synchronize()
+            h0 = time.perf_counter()
             start.record(stream)
             graph.replay()
             end.record(stream)
             torch.cuda.synchronize()
+            walls.append(time.perf_counter() - h0)
             times.append(start.elapsed_time(end) / steps)
-    return statistics.median(times), graph
+    if not detail:
+        return statistics.median(times), graph
+    return {"ms": times[0], "median_ms": statistics.median(times), "min_ms": min(times), "max_ms": max(times),
+            "host_wall_ms": walls[0] * 1e3, "warmup_steps": eager + done * steps,
+            "warmup_s": warm_wall}, graph
 
 
-def cpu_baseline(csr, a_packed, rhs, M, K, N, V, budget_s=12.0):
-    """Restated reference host loop on the same arrays (oracle, checker only)."""
-    import oracle
-    A = np.ascontiguousarray(a_packed).view(np.int64)
-    flops = 2.0 * csr.nnz_vec * V * N
-    # warm-up, then repetitions until the time budget (median reported)
-    oracle.spmm_ref_check(A, 8, rhs, M, K, N, 8, 8, 8, V, csr.row_offsets, csr.col_indices, csr.m_vec, 1)
-    st = []
-    t_end = time.time() + budget_s
-    while len(st) < 5 or (time.time() < t_end and len(st) < 30):
-        t0 = time.perf_counter()
-        oracle.spmm_ref_check(A, 8, rhs, M, K, N, 8, 8, 8, V, csr.row_offsets, csr.col_indices, csr.m_vec, 1)
-        st.append(time.perf_counter() - t0)
-    threads = os.cpu_count() or 1
-    mt = []
-    for _ in range(6):
-        t0 = time.perf_counter()
-        oracle.spmm_ref_check_omp(A, 8, rhs, M, K, N, 8, 8, V, csr.row_offsets, csr.col_indices, csr.m_vec,
-                                  threads)
-        mt.append(time.perf_counter() - t0)
-    st_s, mt_s = statistics.median(st), statistics.median(mt[1:])
-    cpu_model = ""
+def host_cpu():
+    """(model name, threads this process may use) of the host."""
+    model = ""
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
+                    model = line.split(":", 1)[1].strip()
                     break
     except OSError:
         pass
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        threads = os.cpu_count() or 1
+    # the process's share of a shared host, if the environment states it
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        threads = min(threads, int(cap))
+    return model, threads
+
+
+def config1_workload(seed=5):
+    """The reference's CPU-runnable case: A 512 x 512 vector-sparse, V = 8,
+    round(0.1 * 512) = 51 column vectors per 8-row block drawn uniformly
+    without replacement, int8 values uniform in [-128, 127]; B 512 x 128 int8
+    uniform; C = A B in int32."""
+    from magicube_amd import formats, synthetic
+    M = K = 512
+    N, V = 128, 8
+    csr = synthetic.uniform_block_mask(M, K, V, 0.9, seed)
+    a = synthetic.random_lanes(csr.nnz_vec * V, 8, True, seed + 1)
+    b = synthetic.random_lanes(K * N, 8, True, seed + 2)
+    a_packed = formats.pack_lanes(a, 8)
+    return csr, a, b, a_packed, formats.pack_lanes(b, 8).view(np.int32), (M, K, N, V)
+
+
+def _time_host(fn, min_runs, budget_s, max_runs=30):
+    fn()                                            # warm-up
+    ts = []
+    t_end = time.time() + budget_s
+    while len(ts) < min_runs or (time.time() < t_end and len(ts) < max_runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts), len(ts)
+
+
+def cpu_baseline(csr, a_packed, rhs, M, K, N, V, budget_s=10.0):
+    """The reference's host loop compute_ref_integers
+    (SpMM/SpMM/spmm_benchmark.cpp:69-120), restated in oracle/ (checker
+    only), timed on this host: single-threaded (the reference's own CPU
+    path) on the headline arrays, then row-parallel with OpenMP on every
+    thread this process may use, whose C must equal the single-thread C, and
+    single-threaded on the reference's 512 x 512, N = 128 case, whose C must
+    equal the GPU's.  The library built for x86-64-v4 is used where the CPU
+    has AVX-512 (the reference builds -march=native)."""
+    import torch
+    import oracle
+    from magicube_amd import formats, spmm
+    handle, march = oracle.host_lib()
+    model, threads = host_cpu()
+    A = np.ascontiguousarray(a_packed).view(np.int64)
+    flops = 2.0 * csr.nnz_vec * V * N
+    ref = {}
+
+    def single():
+        ref["C"], _ = oracle.spmm_ref_check(A, 8, rhs, M, K, N, 8, 8, 8, V, csr.row_offsets, csr.col_indices,
+                                            csr.m_vec, 1, handle=handle)
+
+    st_s, st_n = _time_host(single, 5, budget_s)
+
+    par = {}
+
+    def parallel():
+        par["C"], _ = oracle.spmm_ref_check_omp(A, 8, rhs, M, K, N, 8, 8, V, csr.row_offsets, csr.col_indices,
+                                                csr.m_vec, threads, handle=handle)
+
+    mt_s, mt_n = _time_host(parallel, 5, 3.0)
+    par_ok = bool(np.array_equal(par["C"], ref["C"]))
+
+    # the reference's CPU-runnable case (512 x 512, V = 8, N = 128), on the GPU too
+    c1, a1, b1, ap1, rhs1, (M1, K1, N1, V1) = config1_workload()
+    A1 = np.ascontiguousarray(ap1).view(np.int64)
+    c1_ref = {}
+
+    def single1():
+        c1_ref["C"], _ = oracle.spmm_ref_check(A1, 8, rhs1, M1, K1, N1, 8, 8, 8, V1, c1.row_offsets,
+                                               c1.col_indices, c1.m_vec, 1, handle=handle)
+
+    s1_s, s1_n = _time_host(single1, 20, 2.0, max_runs=2000)
+    prep1 = formats.prepare_spmm(c1, ap1, 8, 8)
+    dev1 = spmm.DeviceSpmm.upload(prep1)
+    out1 = dev1.matmul(torch.from_numpy(rhs1).cuda(), N1, sign=spmm.SIGNED)
+    torch.cuda.synchronize()
+    # the literal check reads A lanes unsigned and sign-extends the top B
+    # lane (spmm_benchmark.cpp:101,106), so on signed full-range data the GPU
+    # is compared with the plain signed sum
+    plain1 = oracle.spmm_plain(c1.m_vec, V1, N1, c1.row_offsets, c1.col_indices, a1, b1)
+    c1_gpu_exact = bool(np.array_equal(out1.cpu().numpy(), plain1))
+    flops1 = 2.0 * c1.nnz_vec * V1 * N1
     return {
         "value": flops / st_s / 1e12,
         "unit": "TOPS",
         "cores": 1,
         "kind": "port",
-        "sample": f"full workload (same arrays), restated compute_ref_integers single-thread, median of {len(st)} "
-                  f"runs ({st_s * 1e3:.1f} ms each), gcc -O3 -march=x86-64-v3",
-        "parallel": {"value": flops / mt_s / 1e12, "unit": "TOPS", "cores": threads,
-                     "ms": mt_s * 1e3, "kind": "port-openmp"},
-        "cpu_model": cpu_model,
+        "sample": f"headline arrays, restated compute_ref_integers single-thread, median of {st_n} runs "
+                  f"({st_s * 1e3:.0f} ms each), gcc -O3 -march={march}",
+        "verified": par_ok,
+        "cpu_model": model,
+        "parallel": {"value": flops / mt_s / 1e12, "cores": threads, "ms": mt_s * 1e3, "runs": mt_n,
+                     "kind": "port-openmp", "equals_single_thread": par_ok},
+        "config1": {"workload": "512x512 V=8 51 vectors/block, N=128, int8", "nnz_vec": c1.nnz_vec,
+                    "value": flops1 / s1_s / 1e12, "ms": s1_s * 1e3, "runs": s1_n, "cores": 1,
+                    "gpu_equals_plain_sum": c1_gpu_exact},
     }
 
 
@@ -587,11 +674,22 @@ def attention_bench(steps, warmup, seed=101, samples=8, heads=16, V=8, sparsity=
     }
 
 
-def serving_bench(n_requests=48, seed=202, budget_gib=4.0):
+def _outputs_equal(ref, got):
+    return len(ref) == len(got) and all(torch_equal_bits(x, y) for x, y in zip(ref, got))
+
+
+def torch_equal_bits(x, y):
+    import torch
+    return x.shape == y.shape and torch.equal(x.contiguous().view(torch.int32), y.contiguous().view(torch.int32))
+
+
+def serving_trace_bench(n_requests, seed, budget_gib, heads=12, samples=(1, 4), seq=(256, 8192),
+                        workload=""):
     """Continuous-batching server vs per-request launches on one request
-    trace (12 heads x 1-4 samples, L ~ U[256, 8192], s ~ U[0.8, 0.98], int8 or
-    int4 Q/K): saturation throughput (every request queued at t=0) and
-    latency under Poisson arrivals at 60% of the batched capacity."""
+    trace: saturation throughput (every request queued at t=0) and latency
+    under Poisson arrivals at 60% of the batched capacity.  An untimed pass
+    per policy keeps every request's outputs, which must be bitwise equal
+    across the three policies."""
     import math
     import torch
     from magicube_amd import serving
@@ -600,18 +698,28 @@ def serving_bench(n_requests=48, seed=202, budget_gib=4.0):
     for pol in serving.POLICIES:
         serving.Server(budget, pol).run(warm)
         serving.reset(warm)
-    reqs = serving.make_requests(n_requests, math.inf, seed)
+    reqs = serving.make_requests(n_requests, math.inf, seed, heads=heads, samples=samples, seq=seq)
     torch.cuda.synchronize()
-    heads = sum(len(r.matrices) for r in reqs)
-    out = {"workload": "attention requests: 12 heads x 1-4 samples, L~U[256,8192] (x64), s~U[0.8,0.98], "
-                       "int8/int4 Q/K, V=8, row-scaled int8 P",
-           "requests": n_requests, "matrices": heads, "hbm_budget_gib": budget_gib}
+    out = {"workload": workload, "requests": n_requests, "matrices": sum(len(r.matrices) for r in reqs),
+           "hbm_budget_gib": budget_gib}
+    # untimed pass per policy with the outputs kept: allocator growth for the
+    # per-request arenas and first launches are not serving cost, and the
+    # outputs are compared across policies
+    ref, identical = None, True
+    for pol in serving.POLICIES:
+        serving.reset(reqs)
+        serving.Server(budget, pol, keep_outputs=True).run(reqs)
+        torch.cuda.synchronize()
+        got = [[o.clone() for o in r.result()] for r in reqs]
+        if ref is None:
+            ref = got
+        else:
+            identical &= all(_outputs_equal(a, b) for a, b in zip(ref, got))
+        del got
+    del ref
+    out["identical"] = bool(identical)
     sat = {}
     for pol in serving.POLICIES:
-        # one untimed pass per policy on the same trace: allocator growth for
-        # the per-request arenas and first launches are not serving cost
-        serving.reset(reqs)
-        serving.Server(budget, pol).run(reqs)
         serving.reset(reqs)
         sat[pol] = serving.Server(budget, pol).run(reqs)
     out["saturation"] = sat
@@ -626,7 +734,24 @@ def serving_bench(n_requests=48, seed=202, budget_gib=4.0):
         serving.reset(reqs)
         poi[pol] = serving.Server(budget, pol).run(reqs)
     out["poisson"] = {"rate_req_per_s": rate, **poi}
+    serving.reset(reqs)
     return out
+
+
+def serving_bench(n_requests=48, seed=202, budget_gib=4.0):
+    """The serving trace at its stated size: 12 heads x 1-4 samples,
+    L ~ U[256, 8192], s ~ U[0.8, 0.98], int8 or int4 Q/K."""
+    return serving_trace_bench(n_requests, seed, budget_gib,
+                               workload="attention requests: 12 heads x 1-4 samples, L~U[256,8192] (x64), "
+                                        "s~U[0.8,0.98], int8/int4 Q/K, V=8, row-scaled int8 P")
+
+
+def serving_small_bench(n_requests=192, seed=303, budget_gib=4.0):
+    """Small requests, where launch and host work weigh most: 12 heads x 1
+    sample, L ~ U[256, 1024]."""
+    return serving_trace_bench(n_requests, seed, budget_gib, samples=(1, 1), seq=(256, 1024),
+                               workload="attention requests: 12 heads x 1 sample, L~U[256,1024] (x64), "
+                                        "s~U[0.8,0.98], int8/int4 Q/K, V=8, row-scaled int8 P")
 
 
 def time_eager(run, stream, reps=20, warmup=3, rounds=3):
@@ -886,12 +1011,57 @@ def calibrate_copy(mib=64, iters=20):
     print(json.dumps({"calibration_copy_bytes": n * 4, "iters": iters}))
 
 
+def _secondary(sections):
+    """A few figures of the other sections for the one-line summary."""
+    sec = {}
+    rs = sections.get("reference_shape")
+    if rs:
+        sec["spmm_ref_shape_2048x2304x512"] = {"tops": rs["tops"], "x_a100": rs["ratio_vs_a100"]}
+    sd = sections.get("sddmm")
+    if sd:
+        for c in sd["cases"]:
+            if c["bits"] == 8 and c["K"] == 64 and c["sparsity"] == 0.9:
+                sec["sddmm_i8_k64_s090"] = {"us": c["us"], "tops": c["tops"], "frac": c["hbm_frac"],
+                                            "verified": c["verified"]}
+            if c["bits"] == 4 and c["K"] == 128 and c["sparsity"] == 0.95:
+                sec["sddmm_i4_k128_s095"] = {"us": c["us"], "tops": c["tops"], "frac": c["hbm_frac"],
+                                             "verified": c["verified"]}
+    at = sections.get("attention")
+    if at:
+        sec["attention_8x16"] = {"fused_ms": at["fused_graph_ms"], "x_plain_eager": at["speedup_vs_plain_eager"],
+                                 "tops": at["fused_tops"], "fused_equals_plain": at["fused_equals_plain"],
+                                 "rel_fro_fp64_max": at["error_vs_fp64"]["rel_fro_max"]}
+    sv = sections.get("serving")
+    if sv:
+        for name in ("serving", "serving_small"):
+            t = sections.get(name)
+            if not t:
+                continue
+            sat, poi = t["saturation"], t["poisson"]
+            sec[name] = {"req_s": {p: sat[p]["req_per_s"] for p in sat},
+                         "poisson_rate": poi["rate_req_per_s"],
+                         "p50_ms": {p: poi[p]["p50_ms"] for p in sat},
+                         "p99_ms": {p: poi[p]["p99_ms"] for p in sat},
+                         "identical": t.get("identical")}
+    en = sections.get("encoder")
+    if en:
+        for c in en["cases"]:
+            if c.get("batch") == 8 and c.get("lhs_pre") == 8 and c.get("rhs_pre") == 8:
+                sec["encoder_b8_l8r8_ms"] = c["eager_ms"]
+    return sec
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5000,
                     help="untimed steps before the headline timing (also brings the GPU to its working clocks)")
+    ap.add_argument("--warmup-s", type=float, default=0.3,
+                    help="minimum seconds of back-to-back untimed replays before the headline timing, "
+                         "whatever --warmup is")
+    ap.add_argument("--sections-out", default=os.path.join(tempfile.gettempdir(), "magicube_bench_sections.json"),
+                    help="file for the full per-section results (stdout carries one short line)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reference-shape", action="store_true")
     ap.add_argument("--no-attention", action="store_true")
@@ -902,6 +1072,7 @@ def main():
     ap.add_argument("--no-ablation", action="store_true", help="skip the SpMM ablation")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the every-pair sweep and the published-shape SDDMM")
+    ap.add_argument("--headline-only", action="store_true", help="only the headline SpMM (for profiling)")
     ap.add_argument("--config", type=int, default=None, help="explicit SpMM tile config index")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 64 MiB device copy (counter calibration for rocprofv3 --pmc)")
@@ -913,6 +1084,10 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the SpMM would shard by vector-row blocks with B replicated and no exchange)", file=sys.stderr)
         sys.exit(2)
+    if args.headline_only:
+        for k in ("cpu_baseline", "reference_shape", "attention", "serving", "sddmm", "encoder", "vendor",
+                  "ablation", "precision"):
+            setattr(args, "no_" + k, True)
 
     import torch
     from magicube_amd import spmm
@@ -932,12 +1107,16 @@ def main():
         run_once()
     torch.cuda.synchronize()
     verified = checksum_ok(out.cpu().numpy(), csr, a, b, K, N, V)
-    ms, _ = time_gpu(run_once, args.steps, args.warmup, stream)
+    t, _ = time_gpu(run_once, args.steps, args.warmup, stream, warm_s=args.warmup_s, detail=True)
+    ms = t["ms"]
     nnz = csr.nnz_vec
     tops = 2.0 * nnz * V * N / (ms * 1e-3) / 1e12
     alg = algorithmic_bytes(nnz, csr.m_vec, M, K, N, V)
     achieved = alg / (ms * 1e-3) / 1e9
     traffic = load_traffic("spmm_l8r8_m4096_n512_v8_s090_pmc.md")
+    # the result after the timed replays is still the exact product
+    torch.cuda.synchronize()
+    verified = verified and checksum_ok(out.cpu().numpy(), csr, a, b, K, N, V)
 
     result = {
         "metric": "SpMM/SDDMM int8 TOPS & roofline %; sparse-attn req/s with p50/p99 latency",
@@ -952,62 +1131,83 @@ def main():
         "vs_baseline": None,
         "dtype": "int8",
         "data": "synthetic (seeded Poisson vector-sparse mask, uniform full-range signed int8)",
-        "config": {"workload": "spmm_l8r8", "pair": "L8-R8", "signed": True, "M": M, "K": K, "N": N, "V": V,
-                   "sparsity": sparsity, "mask": "poisson_per_block", "seed": seed, "nnz_vec": nnz,
-                   "tile_config": args.config},
+        "config": {"workload": "spmm_l8r8 M=K=4096 N=512 V=8 s=0.9", "pair": "L8-R8", "signed": True,
+                   "M": M, "K": K, "N": N, "V": V, "sparsity": sparsity, "mask": "poisson_per_block",
+                   "seed": seed, "nnz_vec": nnz, "tile_config": args.config},
         "verified": verified,
+        "timing": {"timed_steps": args.steps, "median_of_5_ms": t["median_ms"], "min_ms": t["min_ms"],
+                   "max_ms": t["max_ms"], "host_wall_ms": t["host_wall_ms"], "warmup_steps_run": t["warmup_steps"],
+                   "warmup_s": round(t["warmup_s"], 3)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": alg, "kernel_us": ms * 1e3,
                      "mfma_useful_frac": tops / MFMA_INT8_PEAK_TOPS},
     }
+    sections = {}
+
+    def section(name, fn):
+        # each section's progress on stderr, so a long run shows it is alive
+        t0 = time.perf_counter()
+        print(f"bench: {name} ...", file=sys.stderr, flush=True)
+        sections[name] = fn()
+        print(f"bench: {name} done in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
     if not args.no_reference_shape:
-        # the published A100 shape: 256 vector rows (M = 2048), K = 2304, N = 512, V = 8, s = 0.9
-        rc, ra, rb, rap, rprep, rrhs = make_workload(2048, 2304, 512, 8, 0.9, seed=11)
-        rdev = spmm.DeviceSpmm.upload(rprep)
-        rr = torch.from_numpy(rrhs).cuda()
-        ro = torch.empty((2048, 512), dtype=torch.int32, device="cuda")
+        def ref_shape():
+            # the published A100 shape: 256 vector rows (M = 2048), K = 2304, N = 512, V = 8, s = 0.9
+            rc, ra, rb, rap, rprep, rrhs = make_workload(2048, 2304, 512, 8, 0.9, seed=11)
+            rdev = spmm.DeviceSpmm.upload(rprep)
+            rr = torch.from_numpy(rrhs).cuda()
+            ro = torch.empty((2048, 512), dtype=torch.int32, device="cuda")
 
-        def run_ref():
-            rdev.matmul(rr, 512, out=ro, sign=spmm.SIGNED, stream=stream)
+            def run_ref():
+                rdev.matmul(rr, 512, out=ro, sign=spmm.SIGNED, stream=stream)
 
-        rms, _ = time_gpu(run_ref, args.steps, SECTION_WARMUP, stream)
-        rtops = 2.0 * rc.nnz_vec * 8 * 512 / (rms * 1e-3) / 1e12
-        result["reference_shape"] = {"M": 2048, "K": 2304, "N": 512, "V": 8, "sparsity": 0.9,
-                                     "ms_per_step": rms, "tops": rtops, "a100_published_tops": REFERENCE_A100_TOPS,
-                                     "ratio_vs_a100": rtops / REFERENCE_A100_TOPS,
-                                     "note": "synthetic pattern at the published DLMC shape"}
+            rms, _ = time_gpu(run_ref, args.steps, SECTION_WARMUP, stream)
+            rtops = 2.0 * rc.nnz_vec * 8 * 512 / (rms * 1e-3) / 1e12
+            return {"M": 2048, "K": 2304, "N": 512, "V": 8, "sparsity": 0.9, "ms_per_step": rms, "tops": rtops,
+                    "a100_published_tops": REFERENCE_A100_TOPS, "ratio_vs_a100": rtops / REFERENCE_A100_TOPS,
+                    "note": "synthetic pattern at the published DLMC shape"}
+        section("reference_shape", ref_shape)
 
     if not args.no_sddmm:
-        result["sddmm"] = sddmm_bench(max(1, args.steps // 2), SECTION_WARMUP)
+        section("sddmm", lambda: sddmm_bench(max(1, args.steps // 2), SECTION_WARMUP))
 
     if not args.no_precision:
-        result["precision"] = precision_bench(max(1, args.steps // 2), SECTION_WARMUP)
+        section("precision", lambda: precision_bench(max(1, args.steps // 2), SECTION_WARMUP))
         if not args.no_sddmm:
-            result["sddmm_published_shape"] = sddmm_published_bench(max(1, args.steps // 2),
-                                                                    SECTION_WARMUP)
+            section("sddmm_published_shape", lambda: sddmm_published_bench(max(1, args.steps // 2), SECTION_WARMUP))
 
     if not args.no_attention:
-        result["attention"] = attention_bench(max(1, args.steps // 10), SECTION_WARMUP // 2)
+        section("attention", lambda: attention_bench(max(1, args.steps // 10), SECTION_WARMUP // 2))
 
     if not args.no_serving:
-        result["serving"] = serving_bench()
+        section("serving", serving_bench)
+        section("serving_small", serving_small_bench)
 
     if not args.no_encoder:
-        result["encoder"] = encoder_bench()
+        section("encoder", encoder_bench)
 
     if not args.no_vendor:
-        result["vendor"] = vendor_bench()
-        result["vendor_sweep"] = vendor_sweep_bench()
+        section("vendor", vendor_bench)
+        section("vendor_sweep", vendor_sweep_bench)
 
     if not args.no_ablation:
-        result["ablation"] = ablation_bench(max(1, args.steps // 4), SECTION_WARMUP // 2)
+        section("ablation", lambda: ablation_bench(max(1, args.steps // 4), SECTION_WARMUP // 2))
 
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(csr, a_packed, rhs_np, M, K, N, V)
+        section("cpu_baseline", lambda: cpu_baseline(csr, a_packed, rhs_np, M, K, N, V))
+        result["cpu_baseline"] = sections["cpu_baseline"]
 
-    print(json.dumps(result))
+    result["secondary"] = _secondary(sections)
+    if sections:
+        try:
+            with open(args.sections_out, "w") as f:
+                json.dump({"headline": result, **sections}, f)
+            result["sections_file"] = args.sections_out
+        except OSError as e:
+            result["sections_file"] = f"not written: {e}"
+    print(json.dumps(result, separators=(",", ":")), flush=True)
 
 
 if __name__ == "__main__":

@@ -250,6 +250,27 @@ def tensor_scales(tensors, stream=None) -> list[float]:
     return [127.0 / float(a) if a > 0 else 1.0 for a in amax]
 
 
+# 4-bit Q/K clip at this many RMS: a uniform 16-level quantizer of N(0, 1)
+# data has its smallest squared error near a 2.5 sigma clip, and the layer's
+# error against FP64 attention is 0.16 there, against 0.25 at 7 / max|x|
+# (L = 2048, 80 % sparse, V = 8)
+QK4_CLIP_RMS = 2.5
+
+
+def qkv_scales(mats, qk_bits: int = 8) -> list[tuple]:
+    """Static per-tensor quantization scales of every matrix's (Q, K, V):
+    127 / max|x| for 8-bit operands (and V), 7 / (2.5 rms(x)) for 4-bit Q
+    and K."""
+    flat = tensor_scales([t for m in mats for t in (m.q, m.k, m.v)])
+    if qk_bits == 4:
+        with torch.no_grad():
+            qk = [t for m in mats for t in (m.q, m.k)]
+            rms = torch.stack([t.float().pow(2).mean() for t in qk]).sqrt().cpu().numpy()
+        for n, r in enumerate(rms):
+            flat[3 * (n // 2) + n % 2] = 7.0 / (QK4_CLIP_RMS * float(r)) if r > 0 else 1.0
+    return [tuple(flat[3 * i:3 * i + 3]) for i in range(len(mats))]
+
+
 N_XCD = 8              # MI355X: workgroups are handed to the 8 XCDs in turn
 WAVES_PER_BLOCK = 4    # waves per workgroup of the fused attention launch
 # 0: one wave per work item.  With four waves per SIMD resident, letting the
@@ -328,8 +349,7 @@ class AttentionBatch:
             self.mats = list(resident)
         else:
             if scales is None:
-                flat = tensor_scales([t for m in mats for t in (m.q, m.k, m.v)])
-                scales = [tuple(flat[3 * i:3 * i + 3]) for i in range(len(mats))]
+                scales = qkv_scales(mats, qk_bits)
             outs = outputs if outputs is not None else [None] * len(mats)
             if len(outs) != len(mats):
                 raise ValueError("one output tensor per matrix")

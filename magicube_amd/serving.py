@@ -74,7 +74,8 @@ def make_requests(n: int, rate: float, seed: int, heads: int = 12, samples=(1, 4
     """Synthetic request trace: Poisson arrivals at ``rate`` requests/s (all
     at t=0 when rate is inf), 1-4 samples, L ~ U[256, 8192] in multiples of
     64, sparsity ~ U[0.8, 0.98], int8 or int4 Q/K; inputs N(0, 1) fp16 and
-    masks generated on the device; scales 127 / max|x| per tensor."""
+    masks generated on the device; per-tensor scales from
+    :func:`attention.qkv_scales`."""
     rng = np.random.default_rng(seed)
     gen = torch.Generator(device=device)
     gen.manual_seed(seed)
@@ -92,8 +93,7 @@ def make_requests(n: int, rate: float, seed: int, heads: int = 12, samples=(1, 4
                 q, k, v = (torch.randn((L, attention.HEAD_DIM), generator=gen, device=device,
                                        dtype=torch.float32).half() for _ in range(3))
                 mats.append(attention.AttentionMatrix(q, k, v, synthetic.attention_mask_device(L, V, s, gen, device)))
-        flat = attention.tensor_scales([t for m in mats for t in (m.q, m.k, m.v)])
-        scales = [tuple(flat[3 * i:3 * i + 3]) for i in range(len(mats))]
+        scales = attention.qkv_scales(mats, bits)
         ws = sum(attention.working_set_bytes(m.L, bits) for m in mats)
         reqs.append(Request(rid, float(arrivals[rid]), bits, mats, scales, ws))
     return reqs

@@ -48,17 +48,45 @@ def build() -> str:
     return LIB_PATH
 
 
+LIB_V4_PATH = os.path.join(HERE, "liboracle_v4.so")
+_AVX512 = ("avx512f", "avx512bw", "avx512dq", "avx512vl")
+
+
+def _load(path):
+    handle = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    return handle
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        _lib = ctypes.CDLL(LIB_PATH)
-        for name, (res, args) in _SIGS.items():
-            fn = getattr(_lib, name)
-            fn.restype = res
-            fn.argtypes = args
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+def host_flags() -> set:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+def host_lib():
+    """(library, march): the x86-64-v4 build where this CPU has AVX-512
+    F/BW/DQ/VL, else the x86-64-v3 one.  Used for timing the host loops."""
+    if os.path.exists(LIB_V4_PATH) and all(f in host_flags() for f in _AVX512):
+        return _load(LIB_V4_PATH), "x86-64-v4"
+    return lib(), "x86-64-v3"
 
 
 def _p(a):
@@ -122,22 +150,24 @@ def prepare_values(values: np.ndarray, type_bytes: int, scaleA: int, row_offsets
 
 
 def spmm_ref_check(A: np.ndarray, type_bytes: int, B: np.ndarray, M: int, K: int, N: int, preA: int,
-                   preA_cut: int, preB: int, V: int, row_offsets, col_indices, m_vec: int, scaleA: int):
+                   preA_cut: int, preB: int, V: int, row_offsets, col_indices, m_vec: int, scaleA: int,
+                   handle=None):
     C = np.empty(M * N, dtype=np.int32)
     a = np.ascontiguousarray(A)
     b = _i32(B)
     ro, ci = _i32(row_offsets), _i32(col_indices)
-    flops = lib().orc_spmm_ref_check(_p(a), type_bytes, _p(b), _p(C), M, K, N, preA, preA_cut, preB, V, _p(ro),
+    flops = (handle or lib()).orc_spmm_ref_check(_p(a), type_bytes, _p(b), _p(C), M, K, N, preA, preA_cut, preB, V, _p(ro),
                                      _p(ci), m_vec, scaleA)
     return C.reshape(M, N), flops
 
 
-def spmm_ref_check_omp(A, type_bytes, B, M, K, N, preA, preB, V, row_offsets, col_indices, m_vec, threads=0):
+def spmm_ref_check_omp(A, type_bytes, B, M, K, N, preA, preB, V, row_offsets, col_indices, m_vec, threads=0,
+                       handle=None):
     C = np.empty(M * N, dtype=np.int32)
     a = np.ascontiguousarray(A)
     b = _i32(B)
     ro, ci = _i32(row_offsets), _i32(col_indices)
-    flops = lib().orc_spmm_ref_check_omp(_p(a), type_bytes, _p(b), _p(C), M, K, N, preA, preB, V, _p(ro),
+    flops = (handle or lib()).orc_spmm_ref_check_omp(_p(a), type_bytes, _p(b), _p(C), M, K, N, preA, preB, V, _p(ro),
                                          _p(ci), m_vec, threads)
     return C.reshape(M, N), flops
 

@@ -104,8 +104,9 @@ void orc_spmm_plain(int m_vec, int V, int N, const int *row_offsets, const int *
 void orc_sddmm_plain(int m_vec, int V, int K, const int *aligned_pairs, const int *aligned_cols,
                      const int32_t *lhs, const int32_t *rhs, int32_t *out, int alignment);
 
-/* Row-parallel (OpenMP) copy of orc_spmm_ref_check for TypeA of 4 or 8
- * bytes with scaleA = 1 (oracle_omp.c); threads <= 0 keeps the default. */
+/* Row-parallel (OpenMP) copy of orc_spmm_ref_check for TypeA of 1, 2, 4 or
+ * 8 bytes with scaleA = 1 and preA_cut = preA (oracle_omp.c); threads <= 0
+ * keeps the default. */
 double orc_spmm_ref_check_omp(const void *A, int type_bytes, const int32_t *B, int32_t *C,
                               int M, int K, int N, int preA, int preB, int V,
                               const int *row_offsets, const int *col_indices, int m_vec,

@@ -245,6 +245,55 @@ def test_ref_check_top_lane_quirk(orc):
     assert not np.array_equal(C_ref, orc.spmm_plain(m_vec, V, N, csr.row_offsets, csr.col_indices, a, b))
 
 
+# (preA, preB, V) entries of the dispatch table with one container per vector
+# and no cut: the cases the row-parallel loop handles
+OMP_KEYS = sorted(k for k, (tb, pa, cut, scale) in REF_TABLE.items() if scale == 1 and cut == pa == k[0])
+
+
+@pytest.mark.parametrize("data", ["reference", "full_range"])
+@pytest.mark.parametrize("key", OMP_KEYS)
+def test_ref_check_omp_equals_single_thread(orc, key, data):
+    """The OpenMP loop bench.py times as the parallel CPU baseline gives the
+    literal single-thread check's C and flop count bit for bit, on the
+    reference generator's data and on full-range lanes (where the top B lane
+    of each int32 sign-extends, spmm_benchmark.cpp:106)."""
+    preA, preB, V = key
+    m_vec, K, N = 24, 96, 64
+    tb = REF_TABLE[key][0]
+    if data == "reference":
+        csr, vals, B, _ = _ref_case(orc, preA, preB, V, m_vec, K, N, seed=41)
+    else:
+        csr = small_mask(m_vec, K, V, 0.3, seed=43)
+        count = csr.nnz_vec * V * preA // (tb * 8)
+        rng = np.random.default_rng(44)
+        vals = rng.integers(0, 256, size=count * tb, dtype=np.uint8).view({1: np.int8, 2: np.int16, 4: np.int32,
+                                                                           8: np.int64}[tb])
+        B = rng.integers(0, 256, size=K * N * preB // 8, dtype=np.uint8).view(np.int32)
+    C1, f1 = orc.spmm_ref_check(vals, tb, B, m_vec * V, K, N, preA, preA, preB, V, csr.row_offsets,
+                                csr.col_indices, m_vec, 1)
+    for threads in (1, 3, 8):
+        Cp, fp = orc.spmm_ref_check_omp(vals, tb, B, m_vec * V, K, N, preA, preB, V, csr.row_offsets,
+                                        csr.col_indices, m_vec, threads)
+        assert np.array_equal(Cp, C1), (key, data, threads)
+        assert fp == f1
+    if data == "full_range":
+        # the data reaches the quirk: the all-unsigned product differs
+        a = orc.unpack_lanes(vals.view(np.uint8), preA, csr.nnz_vec * V, signed=False)
+        b = orc.unpack_lanes(B.view(np.uint8), preB, K * N, signed=False)
+        assert not np.array_equal(C1, orc.spmm_plain(m_vec, V, N, csr.row_offsets, csr.col_indices, a, b))
+
+
+def test_ref_check_omp_planted_mistake_is_caught(orc):
+    """A column index off by one in the parallel loop's input changes its C,
+    so the comparison above would catch such a slip."""
+    key = (8, 8, 8)
+    csr, vals, B, _ = _ref_case(orc, 8, 8, 8, 16, 64, 32, seed=45)
+    C1, _ = orc.spmm_ref_check(vals, 8, B, 128, 64, 32, 8, 8, 8, 8, csr.row_offsets, csr.col_indices, 16, 1)
+    bad_ci = (csr.col_indices + 1) % 64
+    Cp, _ = orc.spmm_ref_check_omp(vals, 8, B, 128, 64, 32, 8, 8, 8, csr.row_offsets, bad_ci, 16, 4)
+    assert not np.array_equal(Cp, C1), key
+
+
 @pytest.mark.parametrize("V", [2, 8])
 def test_published_counter_invariant(orc, V):
     # plot/examples/spmm_abl_study.txt:10-14 (V=2) and :22-26 (V=8), L16-R8:

@@ -61,3 +61,36 @@ def test_budget_and_poisson_trace():
     assert stats["p99_ms"] >= stats["p50_ms"] > 0
     with pytest.raises(ValueError):
         serving.Server(budget // 2, "batched").run(reqs)
+
+
+def _same(a, b):
+    return len(a) == len(b) and all(torch.equal(x.view(torch.int32), y.view(torch.int32)) for x, y in zip(a, b))
+
+
+def test_stated_shape_alone_equals_mixed_and_plain():
+    """The serving trace at its stated shape: 12 heads x 1-4 samples,
+    L ~ U[256, 8192], sparsity ~ U[0.8, 0.98], three int8 and three int4
+    requests.  Every request's outputs are bitwise the same served in the
+    mix, computed alone, and from the one-op-per-launch pipeline; the peak
+    working set stays within a budget of two of the largest requests."""
+    reqs = serving.make_requests(3, math.inf, 21, qk_bits=(8,)) + serving.make_requests(3, math.inf, 22, qk_bits=(4,))
+    for n, r in enumerate(reqs):
+        r.rid = n
+    print("trace:", [(r.rid, r.qk_bits, len(r.matrices), max(m.L for m in r.matrices)) for r in reqs], flush=True)
+    assert max(m.L for r in reqs for m in r.matrices) > 4096
+    budget = 2 * max(r.ws_bytes for r in reqs)
+    stats = serving.Server(budget, "batched", keep_outputs=True).run(reqs)
+    torch.cuda.synchronize()
+    assert stats["peak_working_set_bytes"] <= budget
+    served = {r.rid: [o.clone() for o in r.result()] for r in reqs}
+    for r in reqs:
+        serving.reset([r])
+        serving.Server(budget, "fused", keep_outputs=True).run([r])
+        torch.cuda.synchronize()
+        assert _same(r.result(), served[r.rid]), f"request {r.rid} differs alone vs in the mix"
+    serving.reset(reqs)
+    serving.Server(budget, "plain", keep_outputs=True).run(reqs)
+    torch.cuda.synchronize()
+    for r in reqs:
+        assert _same(r.result(), served[r.rid]), f"request {r.rid} differs batched vs plain"
+    serving.reset(reqs)
