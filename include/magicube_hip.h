@@ -2,8 +2,7 @@
  * magicube_hip.h — C ABI of the MI355X (gfx950) quantized vector-sparse
  * SpMM / SDDMM library.  Plain pointers and sizes only; every device pointer
  * is caller-allocated HBM; the library allocates nothing in these calls and
- * keeps no state between them (except the tile-config table, see
- * mc_spmm_set_tile_config).  All launches are asynchronous on `stream`
+ * keeps no state between them.  All launches are asynchronous on `stream`
  * (0 = the null stream) and never synchronize, so they can be captured in a
  * hipGraph.
  *
@@ -132,9 +131,20 @@ mc_status mc_spmm_run(int lhs_bits, int rhs_bits, mc_signedness sign, int config
                       int vec_length, int n, int k, const int *row_indices,
                       const int *row_offsets, const int *column_indices, const int *values,
                       const int *rhs_matrix, int *output_matrix, hipStream_t stream);
-/* Installs the config the named entry points use for (lhs_bits, rhs_bits,
- * vec_length, n) — the autotuner's result; -1 restores the default. */
-mc_status mc_spmm_set_tile_config(int lhs_bits, int rhs_bits, int vec_length, int n, int config);
+/* Batched SpMM over `batch` matrices sharing one mask (row_indices,
+ * row_offsets, column_indices): matrix b's prepared values start
+ * b * values_stride_bytes after `values`, its B b * rhs_stride_bytes after
+ * `rhs_matrix` and its C b * out_stride int32 after `output_matrix`.  One
+ * launch, matrices in gridDim.y (batch <= 65535); gather tile configs only
+ * (B-stationary configs return MC_EUNSUPPORTED).  Replaces the batched
+ * launchers behind the reference's bspmm_8b / _16b8b / _8b4b / _4b
+ * (end2end_eval/sparse_transformer_magicube/src/cuda/deq_spmm.cpp:5-141,
+ * deq_spmm_kernel.cu:1287-1361), which launch the batch as one grid. */
+mc_status mc_spmm_batched(int lhs_bits, int rhs_bits, mc_signedness sign, int config, int batch, int m_vec,
+                          int vec_length, int n, int k, const int *row_indices, const int *row_offsets,
+                          const int *column_indices, const int *values, long long values_stride_bytes,
+                          const int *rhs_matrix, long long rhs_stride_bytes, int *output_matrix,
+                          long long out_stride, hipStream_t stream);
 
 /* ---- SDDMM ----------------------------------------------------------------
  * Replace sddmm::wmmaSddmm_8b / _4b / _16b (SDDMM/SDDMM/include/wmma_sddmm.cuh:7-32,
@@ -164,6 +174,17 @@ mc_status mc_sddmm_run(int bits, mc_signedness sign, int layout, int m_vec, int 
                        const int *row_indices, const int *row_offsets, const int *col_indices,
                        const int *lhs_matrix, const int *rhs_matrix, int *output_values,
                        int vec_length, hipStream_t stream);
+/* Batched SDDMM over `batch` matrices sharing one mask: matrix b's lhs and
+ * rhs start b * lhs_stride_bytes / b * rhs_stride_bytes after the first,
+ * its output b * out_stride int32 after.  One launch (batch <= 65535).
+ * Replaces the batched launcher behind the reference's bsddmm_8b / _4b
+ * (end2end_eval/sparse_transformer_magicube/src/cuda/deq_sddmm.cpp:56-93,
+ * deq_sddmm_kernel.cu:565-636). */
+mc_status mc_sddmm_batched(int bits, mc_signedness sign, int layout, int batch, int m_vec, int k, int n,
+                           const int *row_indices, const int *row_offsets, const int *col_indices,
+                           const int *lhs_matrix, long long lhs_stride_bytes, const int *rhs_matrix,
+                           long long rhs_stride_bytes, int *output_values, long long out_stride, int vec_length,
+                           hipStream_t stream);
 
 /* ---- Quantized sparse attention ------------------------------------------
  * Layer semantics of spMultiheadAttention
@@ -229,6 +250,29 @@ mc_status mc_attn_softmax_requant(int m_vec, int vec_length, const int *row_indi
  * the dequantizing epilogue of the reference's bspmm (deq_spmm_kernel.cu:645-670). */
 mc_status mc_attn_dequant(const int *in, int rows, int cols, float scale, const float *row_scale,
                           float *out, hipStream_t stream);
+/* out_f16[i] = half(float(in[i]) / divisor), round to nearest even: the
+ * reference's fp16 epilogues of bsddmm (sddmm_utils/output_tile.h:37-43) and
+ * bspmm (deq_spmm_kernel.cu:645-670). */
+mc_status mc_attn_dequant_f16(const int *in, long long count, float divisor, void *out_f16,
+                              hipStream_t stream);
+/* The reference's q_bcsr_softmax (src/cuda/q_softmax.cpp:26-52,
+ * q_softmax_kernel.cu:180-271,394-479) on fp16 logits, for `batch` matrices
+ * sharing one mask: values_f16 holds matrix b's [slot][V] logits at
+ * b * values_stride halves; vector row m = row_indices[x] (x < rows; NULL:
+ * x) gets
+ *   s = float(logit) * sqrt_dk,  max_v from -1e10 (v = 0) or 0 (v >= 1),
+ *   P = (int) clamp(exp(s - max_v) / sum_v * scale, lo, hi)
+ * at bits 16 / 8 / 4 in the reference's layouts, matrix b at attn +
+ * b * attn_stride_bytes + start * V * bits / 8:
+ *   8-bit  [(i/16)][v][16] bytes; 16-bit [(i/16)][v][low 16][high 16];
+ *   4-bit  [(i/32)][v][16] bytes of nibble pairs (slot 2j low, 2j+1 high).
+ * Every slot of [start, start + roundup(nonzeros, 32 for 4-bit, else 16))
+ * is written (padding with 0).  vec_length 8 needs 16-byte aligned logits
+ * (values_f16 and values_stride a multiple of 8). */
+mc_status mc_attn_softmax_f16(int batch, int rows, int vec_length, const int *row_indices,
+                              const int *row_offsets, const void *values_f16, long long values_stride,
+                              float sqrt_dk, float scale, int bits, void *attn, long long attn_stride_bytes,
+                              hipStream_t stream);
 
 /* One attention matrix of the fused batched launch (device-resident
  * descriptor).  q, k: [L][64] int8 (or int4 packed [L][32] bytes); v: [L][64]

@@ -34,7 +34,6 @@ _SIGS = {
     "mc_last_hip_error": (_I, []),
     "mc_spmm_num_tile_configs": (_I, []),
     "mc_spmm_get_tile_config": (_I, [_I, _P]),
-    "mc_spmm_set_tile_config": (_I, [_I, _I, _I, _I, _I]),
     "mc_spmm_run": (_I, [_I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "mc_format_row_alignment": (_I, [_I, _I]),
     "mc_format_align_rows": (_I, [_I, _P, _I, _P]),
@@ -53,6 +52,10 @@ _SIGS["mc_attn_absmax"] = (_I, [_P, _LL, _P, _P])
 _SIGS["mc_attn_softmax_requant"] = (_I, [_I, _I, _P, _P, _P, _F, _F, _I, _P, _P, _P])
 _SIGS["mc_attn_dequant"] = (_I, [_P, _I, _I, _F, _P, _P, _P])
 _SIGS["mc_attn_forward"] = (_I, [_P, _P, _I, _I, _I, _F, _I, _I, _I, _P])
+_SIGS["mc_attn_dequant_f16"] = (_I, [_P, _LL, _F, _P, _P])
+_SIGS["mc_attn_softmax_f16"] = (_I, [_I, _I, _I, _P, _P, _P, _LL, _F, _F, _I, _P, _LL, _P])
+_SIGS["mc_spmm_batched"] = (_I, [_I, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _LL, _P, _LL, _P, _LL, _P])
+_SIGS["mc_sddmm_batched"] = (_I, [_I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _LL, _P, _LL, _P, _LL, _I, _P])
 for _name in ("mc_spmm_l8r8", "mc_spmm_l16r8", "mc_spmm_l8r4", "mc_spmm_l4r4", "mc_spmm_l12r8", "mc_spmm_l16r16",
               "mc_spmm_l12r4", "mc_spmm_l16r4"):
     _SIGS[_name] = (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P])

@@ -13,8 +13,9 @@ descriptors of the fused launch.  It runs
   matrix (quantize x3, SDDMM, softmax + requantize, SpMM, dequantize).
 
 Both give identical int32 logits, int8 probabilities and int32 P V.
-Reference-named op mirrors (``bquantization``, ``q_bcsr_softmax`` ...) are
-at the end of the module.
+:func:`bquantization` and the int32-logit :func:`softmax_requant` sit at the
+end of the module; the reference's op signatures (``bsddmm_8b``,
+``q_bcsr_softmax``, ``bspmm_8b`` ...) are in :mod:`magicube_amd.sptrans`.
 """
 from __future__ import annotations
 
@@ -605,8 +606,15 @@ class BlockBatch:
         order = interleave_bins(lists)
         self.n_items = int(len(order)) if sum(b.n_items for b in blocks) else 0
         self.items = _pinned(order).to(dev, non_blocking=True)
+        # the descriptor copies run on the current stream; launches on any
+        # other stream wait for them
+        self._ready = torch.cuda.Event() if dev.type == "cuda" else None
+        if self._ready is not None:
+            self._ready.record()
 
     def forward(self, stream=None, grid_waves: int = DEFAULT_GRID_WAVES) -> None:
+        if self._ready is not None and not torch.cuda.is_current_stream_capturing():
+            (stream if stream is not None else torch.cuda.current_stream()).wait_event(self._ready)
         lib, st = _lib.lib(), _stream(stream)
         _lib.check(lib.mc_attn_quantize_batched(_dptr(self.qk_jobs), 2 * self.n_mats, self.qk_bits, self.rounding,
                                                 self.max_count, st), "mc_attn_quantize_batched")
@@ -617,33 +625,44 @@ class BlockBatch:
 
 
 # ---------------------------------------------------------------------------
-# Reference-named op mirrors (src/cuda/quantization.cpp, q_softmax.cpp,
-# deq_sddmm.cpp, deq_spmm.cpp): same roles, device tensors in, tensors out.
+# Single ops on device tensors: the quantizer (src/cuda/quantization.cpp)
+# and the softmax + requantization of int32 logits.
 # ---------------------------------------------------------------------------
 
 def bquantization(x: torch.Tensor, bits: int, scale: float, rounding: int = ROUND_TRUNC,
                   stream=None) -> torch.Tensor:
     """fp16 [..., L, D] -> int32-packed [..., L, D*bits/32] (trunc(clamp(x*scale)) by default).
-    D * bits must be a multiple of 32 (the reference passes D = 64)."""
+    D * bits must be a multiple of 32 (the reference passes D = 64).
+
+    Runs on ``stream`` (default: the current stream), which must see ``x``
+    ready.  The contiguous copy of ``x``, the job descriptor and the output
+    are made on that stream too, so the caching allocator cannot hand their
+    memory to other work before the launch has read or written it."""
     if bits not in (8, 4):
         raise ValueError("bits must be 8 or 4")
     if x.dim() == 0 or (x.shape[-1] * bits) % 32 != 0:
         raise ValueError(f"the last dimension ({x.shape[-1] if x.dim() else 0}) times bits must be a multiple of 32")
-    x = x.contiguous()
-    out = torch.empty(x.shape[:-1] + (x.shape[-1] * bits // 32,), dtype=torch.int32, device=x.device)
-    job = _struct_tensor([_QuantJobC(x.data_ptr(), x.numel(), float(scale), out.data_ptr())], x.device)
-    _lib.check(_lib.lib().mc_attn_quantize_batched(_dptr(job), 1, bits, rounding, x.numel(), _stream(stream)),
-               "bquantization")
+    if not x.is_cuda or x.dtype != torch.float16:
+        raise ValueError("x must be a float16 device tensor")
+    ctx = torch.cuda.stream(stream) if isinstance(stream, torch.cuda.Stream) else contextlib.nullcontext()
+    with ctx:
+        x = x.contiguous()
+        out = torch.empty(x.shape[:-1] + (x.shape[-1] * bits // 32,), dtype=torch.int32, device=x.device)
+        job = _struct_tensor([_QuantJobC(x.data_ptr(), x.numel(), float(scale), out.data_ptr())], x.device)
+        _lib.check(_lib.lib().mc_attn_quantize_batched(_dptr(job), 1, bits, rounding, x.numel(), _stream(stream)),
+                   "bquantization")
     return out
 
 
-def q_bcsr_softmax(row_indices, row_offsets, s_vals, m_vec: int, vec_length: int, alpha: float, p_scale: float,
-                   p_mode: int = P_REFERENCE, aligned_num: int | None = None, stream=None):
-    """int32 logits [slot][V] -> int8 P in the SpMM A layout (+ 1/sum per row)."""
+def softmax_requant(row_indices, row_offsets, s_vals, m_vec: int, vec_length: int, alpha: float, p_scale: float,
+                    p_mode: int = P_REFERENCE, aligned_num: int | None = None, stream=None):
+    """int32 logits [slot][V] -> int8 P in the SpMM A layout (+ 1/sum per row).
+    The reference's fp16-logit op with its signature is
+    :func:`magicube_amd.sptrans.q_bcsr_softmax`."""
     n = aligned_num if aligned_num is not None else s_vals.numel() // vec_length
     p = torch.zeros(max(n * vec_length, 16), dtype=torch.uint8, device=s_vals.device)
     inv_l = torch.empty(max(m_vec * vec_length, 1), dtype=torch.float32, device=s_vals.device)
     _lib.check(_lib.lib().mc_attn_softmax_requant(m_vec, vec_length, _dptr(row_indices), _dptr(row_offsets),
                                                   _dptr(s_vals), alpha, p_scale, p_mode, _dptr(p), _dptr(inv_l),
-                                                  _stream(stream)), "q_bcsr_softmax")
+                                                  _stream(stream)), "softmax_requant")
     return p, inv_l

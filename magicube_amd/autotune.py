@@ -15,8 +15,9 @@ fastest:
   ``~/.cache/magicube_amd/tune.json``) under a key of device architecture,
   pair, signedness, V, N and log2 buckets of m_vec and of the mean nonzero
   vectors per row;
-* it is installed with ``mc_spmm_set_tile_config``, so the named entry
-  points (``mc_spmm_l8r8`` ...) use it for that (pair, V, N).
+* it is installed on the tuned operand (``DeviceSpmm.configs[N]``), whose
+  products with that N then name it in every call; the C entry points keep
+  no state, so other callers in the process are unaffected.
 
 :func:`tune_attention` picks the persistent grid size of the fused
 attention launch the same way.  The reference has no counterpart: it fixes
@@ -143,7 +144,7 @@ def tune_spmm(dev: spmm.DeviceSpmm, rhs: torch.Tensor, n: int, sign: int = spmm.
             entries[key] = {"config": best, "us": timings[best], "timings": {str(k): v for k, v in timings.items()}}
             save_cache(entries, cache_file)
     if install:
-        spmm.set_tile_config(p.lhs_bits, p.rhs_bits, p.V, n, res.config)
+        dev.configs[n] = res.config
     return res
 
 

@@ -197,6 +197,134 @@ __global__ __launch_bounds__(64 * WPB) void softmax_requant_kernel(int m_vec, co
     }
 }
 
+// dequantize to fp16: out[i] = half(float(in[i]) / divisor), round to nearest
+// even (the reference's epilogues, deq_spmm_kernel.cu:645-670 and
+// sddmm_utils/output_tile.h:37-43: __float2half((float)acc / scale))
+__global__ __launch_bounds__(256) void dequant_f16_kernel(const int *in, long long count, float divisor,
+                                                          __half *out) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < count; i += (long long)gridDim.x * 256)
+        out[i] = __float2half_rn((float)in[i] / divisor);
+}
+
+// ---------------------------------------------------------------------------
+// The reference's batched quantized softmax on fp16 logits
+// (q_softmax_kernel.cu:180-271, host :394-479): for matrix b of a batch
+// sharing one mask and each vector row m = row_indices[x],
+//   s = float(values[slot][v]) * sqrt_dk
+//   max_v = max over the row's nonzeros of s, starting from -1e10 for v = 0
+//           and from 0 for v >= 1 (the reference's register init sets only
+//           element 0, :209)
+//   sum_v = sum of exp(s - max_v)
+//   P = (char / short) clamp(exp(s - max_v) / sum_v * scale, lo, hi)
+// written at the reference's layouts (attn + start * V * bits / 8 bytes):
+//   8-bit   [(i/16)][v][16] bytes
+//   16-bit  [(i/16)][v][low 16 bytes][high 16 bytes]
+//   4-bit   [(i/32)][v][16 bytes], byte j = slot 2j (low nibble) | 2j+1
+// Every slot of [start, start + roundup(nonzeros, A)) is written (A = 32 for
+// 4-bit P, else 16), padding slots with 0.  The sum runs in a fixed order:
+// lane l adds slots l, l + 64, ..., then a butterfly over the 64 lanes.
+// ---------------------------------------------------------------------------
+template <int V>
+__device__ __forceinline__ void load_half_vec(const __half *p, float (&x)[V]) {
+    if constexpr (V == 8) {
+        const v4i w = *reinterpret_cast<const v4i *>(p);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int we = w[e];
+            const __half2 h2 = *reinterpret_cast<const __half2 *>(&we);
+            x[2 * e] = __low2float(h2);
+            x[2 * e + 1] = __high2float(h2);
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v) x[v] = __half2float(p[v]);
+    }
+}
+
+__device__ __forceinline__ float wave_max(float x) {
+    for (int s = 32; s > 0; s >>= 1) x = fmaxf(x, __shfl_xor(x, s, 64));
+    return x;
+}
+__device__ __forceinline__ float wave_sum(float x) {
+    for (int s = 1; s < 64; s <<= 1) x += __shfl_xor(x, s, 64);
+    return x;
+}
+
+template <int V, int BITS, int WPB>
+__global__ __launch_bounds__(64 * WPB) void ref_softmax_kernel(int rows, const int *row_indices, const int *row_offsets,
+                                                               const __half *values, long long values_stride,
+                                                               float sqrt_dk, float scale, unsigned char *attn,
+                                                               long long attn_stride) {
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int x = blockIdx.x * WPB + wave;
+    if (x >= rows) return;
+    const long long b = blockIdx.y;
+    const int m = uniform(row_indices ? row_indices[x] : x);
+    const int start = uniform(row_offsets[2 * m]), nz = uniform(row_offsets[2 * m + 1]) - start;
+    const __half *val = values + b * values_stride + (size_t)start * V;
+    unsigned char *out = attn + b * attn_stride + (size_t)start * V * BITS / 8;
+
+    float mx[V], sm[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) { mx[v] = v == 0 ? -1e10f : 0.f; sm[v] = 0.f; }
+    for (int i = lane; i < nz; i += 64) {
+        float s[V];
+        load_half_vec<V>(val + (size_t)i * V, s);
+#pragma unroll
+        for (int v = 0; v < V; ++v) mx[v] = fmaxf(mx[v], s[v] * sqrt_dk);
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) mx[v] = wave_max(mx[v]);
+    for (int i = lane; i < nz; i += 64) {
+        float s[V];
+        load_half_vec<V>(val + (size_t)i * V, s);
+#pragma unroll
+        for (int v = 0; v < V; ++v) sm[v] += __expf(s[v] * sqrt_dk - mx[v]);
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) sm[v] = wave_sum(sm[v]);
+
+    auto prob = [&](int i, int v, const float (&s)[V], float lo, float hi) {
+        if (i >= nz) return 0.f;
+        const float p = __expf(s[v] * sqrt_dk - mx[v]) / sm[v] * scale;
+        return fminf(fmaxf(p, lo), hi);
+    };
+    if constexpr (BITS == 4) {
+        // slot pairs (2j, 2j + 1) of 32-slot groups
+        const int npair = ((nz + 31) & ~31) / 2;
+        for (int j = lane; j < npair; j += 64) {
+            const int i = 2 * j;
+            float s0[V], s1[V];
+            if (i < nz) load_half_vec<V>(val + (size_t)i * V, s0);
+            if (i + 1 < nz) load_half_vec<V>(val + (size_t)(i + 1) * V, s1);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const int p0 = (int)(signed char)prob(i, v, s0, -8.f, 7.f);
+                const int p1 = (int)(signed char)prob(i + 1, v, s1, -8.f, 7.f);
+                out[(size_t)(i >> 5) * V * 16 + 16 * v + (j & 15)] = (unsigned char)((p0 & 15) | (p1 << 4));
+            }
+        }
+    } else {
+        const int nw = (nz + 15) & ~15;
+        for (int i = lane; i < nw; i += 64) {
+            float s[V];
+            if (i < nz) load_half_vec<V>(val + (size_t)i * V, s);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                if constexpr (BITS == 8) {
+                    const int p = (int)(signed char)prob(i, v, s, -128.f, 127.f);
+                    out[(size_t)(i >> 4) * V * 16 + 16 * v + (i & 15)] = (unsigned char)p;
+                } else {
+                    const int p = (int)(short)prob(i, v, s, -32768.f, 32767.f);
+                    unsigned char *o = out + (size_t)(i >> 4) * V * 32 + 32 * v + (i & 15);
+                    o[0] = (unsigned char)(p & 255);
+                    o[16] = (unsigned char)((p >> 8) & 255);
+                }
+            }
+        }
+    }
+}
+
 // Row bounds are read-only for the whole launch; read through the constant
 // address space at a wave-uniform address they become scalar loads (scalar
 // cache, lgkmcnt), which neither wait behind nor hold back vector gathers.
@@ -524,6 +652,47 @@ mc_status mc_attn_dequant(const int *in, int rows, int cols, float scale, const 
     if (!in || !out) return MC_EINVAL_ARG;
     hipLaunchKernelGGL(mc::dequant_kernel, dim3(grid_for(count, 256 * 8)), dim3(256), 0, stream, in, count, cols,
                        scale, row_scale, out);
+    return mc::check_launch();
+}
+
+mc_status mc_attn_dequant_f16(const int *in, long long count, float divisor, void *out_f16, hipStream_t stream) {
+    if (count < 0) return MC_EINVAL_SHAPE;
+    if (count == 0) return MC_OK;
+    if (!in || !out_f16) return MC_EINVAL_ARG;
+    hipLaunchKernelGGL(mc::dequant_f16_kernel, dim3(grid_for(count, 256 * 8)), dim3(256), 0, stream, in, count,
+                       divisor, static_cast<__half *>(out_f16));
+    return mc::check_launch();
+}
+
+mc_status mc_attn_softmax_f16(int batch, int rows, int vec_length, const int *row_indices, const int *row_offsets,
+                              const void *values_f16, long long values_stride, float sqrt_dk, float scale, int bits,
+                              void *attn, long long attn_stride_bytes, hipStream_t stream) {
+    if (bits != 16 && bits != 8 && bits != 4) return MC_EUNSUPPORTED;
+    if (!mc::valid_v(vec_length)) return MC_EINVAL_V;
+    if (rows < 0 || batch < 1 || batch > 65535 || values_stride < 0 || attn_stride_bytes < 0) return MC_EINVAL_SHAPE;
+    if (rows == 0) return MC_OK;
+    if (!row_offsets || !values_f16 || !attn) return MC_EINVAL_ARG;
+    // 16-byte loads of the eight fp16 logits of a slot
+    if (vec_length == 8 && ((reinterpret_cast<uintptr_t>(values_f16) & 15u) != 0 || values_stride % 8 != 0))
+        return MC_EINVAL_ARG;
+    constexpr int WPB = 4;
+    const dim3 grid((unsigned)((rows + WPB - 1) / WPB), (unsigned)batch), block(64 * WPB);
+    const auto *vals = static_cast<const __half *>(values_f16);
+    auto *out = static_cast<unsigned char *>(attn);
+#define MC_REF_SOFTMAX(V_, B_)                                                                                     \
+    hipLaunchKernelGGL((mc::ref_softmax_kernel<V_, B_, WPB>), grid, block, 0, stream, rows, row_indices,           \
+                       row_offsets, vals, values_stride, sqrt_dk, scale, out, attn_stride_bytes)
+#define MC_REF_SOFTMAX_V(B_)                                                                                       \
+    switch (vec_length) {                                                                                          \
+    case 2: MC_REF_SOFTMAX(2, B_); break;                                                                          \
+    case 4: MC_REF_SOFTMAX(4, B_); break;                                                                          \
+    default: MC_REF_SOFTMAX(8, B_); break;                                                                         \
+    }
+    if (bits == 16) { MC_REF_SOFTMAX_V(16) }
+    else if (bits == 8) { MC_REF_SOFTMAX_V(8) }
+    else { MC_REF_SOFTMAX_V(4) }
+#undef MC_REF_SOFTMAX_V
+#undef MC_REF_SOFTMAX
     return mc::check_launch();
 }
 

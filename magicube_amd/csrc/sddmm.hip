@@ -12,44 +12,49 @@ namespace {
 // M = N = 4096, V = 8: K = 64 5 % faster with 16 waves, K = 128 and 512
 // 7-11 % slower.
 template <int V, int BITS, bool SGN, int NKC, bool REF>
-mc_status sddmm_launch(const SddmmArgs &a, hipStream_t s) {
+mc_status sddmm_launch(const SddmmArgs &a, int batch, hipStream_t s) {
     constexpr int W = NKC == 1 ? 16 : 8;
     constexpr int TPW = NKC == 1 ? 1 : (NKC >= 4 ? 1 : 4 / NKC);
-    hipLaunchKernelGGL((sddmm_kernel<V, BITS, SGN, NKC, W, TPW, REF>), dim3((unsigned)a.m_vec), dim3(64 * W), 0, s,
-                       a);
+    hipLaunchKernelGGL((sddmm_kernel<V, BITS, SGN, NKC, W, TPW, REF>), dim3((unsigned)a.m_vec, (unsigned)batch),
+                       dim3(64 * W), 0, s, a);
     return check_launch();
 }
 
 template <int V, int BITS, bool SGN, bool REF>
-mc_status sddmm_launch_k(int nkc, const SddmmArgs &a, hipStream_t s) {
+mc_status sddmm_launch_k(int nkc, const SddmmArgs &a, int batch, hipStream_t s) {
     switch (nkc) {
-    case 1: return sddmm_launch<V, BITS, SGN, 1, REF>(a, s);
-    case 2: return sddmm_launch<V, BITS, SGN, 2, REF>(a, s);
-    case 4: return sddmm_launch<V, BITS, SGN, 4, REF>(a, s);
-    case 8: return sddmm_launch<V, BITS, SGN, 8, REF>(a, s);
+    case 1: return sddmm_launch<V, BITS, SGN, 1, REF>(a, batch, s);
+    case 2: return sddmm_launch<V, BITS, SGN, 2, REF>(a, batch, s);
+    case 4: return sddmm_launch<V, BITS, SGN, 4, REF>(a, batch, s);
+    case 8: return sddmm_launch<V, BITS, SGN, 8, REF>(a, batch, s);
     default: return MC_EUNSUPPORTED;
     }
 }
 
 template <int BITS, bool SGN, bool REF>
-mc_status sddmm_launch_v(int V, int nkc, const SddmmArgs &a, hipStream_t s) {
+mc_status sddmm_launch_v(int V, int nkc, const SddmmArgs &a, int batch, hipStream_t s) {
     switch (V) {
-    case 2: return sddmm_launch_k<2, BITS, SGN, REF>(nkc, a, s);
-    case 4: return sddmm_launch_k<4, BITS, SGN, REF>(nkc, a, s);
-    case 8: return sddmm_launch_k<8, BITS, SGN, REF>(nkc, a, s);
+    case 2: return sddmm_launch_k<2, BITS, SGN, REF>(nkc, a, batch, s);
+    case 4: return sddmm_launch_k<4, BITS, SGN, REF>(nkc, a, batch, s);
+    case 8: return sddmm_launch_k<8, BITS, SGN, REF>(nkc, a, batch, s);
     default: return MC_EINVAL_V;
     }
 }
 
 template <int BITS>
-mc_status sddmm_launch_all(bool sgn, bool ref, int V, int nkc, const SddmmArgs &a, hipStream_t s) {
-    if (sgn) return ref ? sddmm_launch_v<BITS, true, true>(V, nkc, a, s) : sddmm_launch_v<BITS, true, false>(V, nkc, a, s);
-    return ref ? sddmm_launch_v<BITS, false, true>(V, nkc, a, s) : sddmm_launch_v<BITS, false, false>(V, nkc, a, s);
+mc_status sddmm_launch_all(bool sgn, bool ref, int V, int nkc, const SddmmArgs &a, int batch, hipStream_t s) {
+    if (sgn)
+        return ref ? sddmm_launch_v<BITS, true, true>(V, nkc, a, batch, s)
+                   : sddmm_launch_v<BITS, true, false>(V, nkc, a, batch, s);
+    return ref ? sddmm_launch_v<BITS, false, true>(V, nkc, a, batch, s)
+               : sddmm_launch_v<BITS, false, false>(V, nkc, a, batch, s);
 }
 
 mc_status sddmm_entry(int bits, mc_signedness sign, int layout, int m_vec, int k, int n, const int *row_indices,
                       const int *row_offsets, const int *col_indices, const int *lhs_matrix, const int *rhs_matrix,
-                      int *output_values, int vec_length, hipStream_t stream) {
+                      int *output_values, int vec_length, hipStream_t stream, int batch = 1,
+                      long long lhs_stride = 0, long long rhs_stride = 0, long long out_stride = 0) {
+    if (batch < 1 || batch > 65535 || lhs_stride < 0 || rhs_stride < 0 || out_stride < 0) return MC_EINVAL_SHAPE;
     if (bits != 16 && bits != 8 && bits != 4) return MC_EUNSUPPORTED;
     if (sign != MC_SIGNED && sign != MC_UNSIGNED) return MC_EINVAL_ARG;
     if (layout != MC_SDDMM_OUT_REFERENCE && layout != MC_SDDMM_OUT_VECTOR) return MC_EINVAL_ARG;
@@ -76,10 +81,13 @@ mc_status sddmm_entry(int bits, mc_signedness sign, int layout, int m_vec, int k
     a.out = output_values;
     a.lhs_bytes = (unsigned)lhs_bytes;
     a.rhs_bytes = (unsigned)rhs_bytes;
+    a.lhs_stride = lhs_stride;
+    a.rhs_stride = rhs_stride;
+    a.out_stride = out_stride;
     const bool sgn = sign == MC_SIGNED, ref = layout == MC_SDDMM_OUT_REFERENCE;
-    if (bits == 16) return sddmm_launch_all<16>(sgn, ref, vec_length, nkc, a, stream);
-    return bits == 8 ? sddmm_launch_all<8>(sgn, ref, vec_length, nkc, a, stream)
-                     : sddmm_launch_all<4>(sgn, ref, vec_length, nkc, a, stream);
+    if (bits == 16) return sddmm_launch_all<16>(sgn, ref, vec_length, nkc, a, batch, stream);
+    return bits == 8 ? sddmm_launch_all<8>(sgn, ref, vec_length, nkc, a, batch, stream)
+                     : sddmm_launch_all<4>(sgn, ref, vec_length, nkc, a, batch, stream);
 }
 
 }  // namespace
@@ -113,6 +121,16 @@ mc_status mc_sddmm_run(int bits, mc_signedness sign, int layout, int m_vec, int 
                        const int *rhs_matrix, int *output_values, int vec_length, hipStream_t stream) {
     return mc::sddmm_entry(bits, sign, layout, m_vec, k, n, row_indices, row_offsets, col_indices, lhs_matrix,
                            rhs_matrix, output_values, vec_length, stream);
+}
+
+mc_status mc_sddmm_batched(int bits, mc_signedness sign, int layout, int batch, int m_vec, int k, int n,
+                           const int *row_indices, const int *row_offsets, const int *col_indices,
+                           const int *lhs_matrix, long long lhs_stride_bytes, const int *rhs_matrix,
+                           long long rhs_stride_bytes, int *output_values, long long out_stride, int vec_length,
+                           hipStream_t stream) {
+    return mc::sddmm_entry(bits, sign, layout, m_vec, k, n, row_indices, row_offsets, col_indices, lhs_matrix,
+                           rhs_matrix, output_values, vec_length, stream, batch, lhs_stride_bytes,
+                           rhs_stride_bytes, out_stride);
 }
 
 }  // extern "C"

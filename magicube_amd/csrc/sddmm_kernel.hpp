@@ -45,6 +45,12 @@ struct SddmmArgs {
     int *out;
     unsigned lhs_bytes;
     unsigned rhs_bytes;        // < 2^30
+    // batched launch (blockIdx.y = matrix b of one shared mask): matrix b's
+    // lhs / rhs start lhs_stride / rhs_stride bytes after matrix b-1's, its
+    // output out_stride int32 after; 0 for a single matrix
+    long long lhs_stride;
+    long long rhs_stride;
+    long long out_stride;
 };
 
 template <int BITS, bool SGN> struct SddmmDigits;
@@ -120,8 +126,10 @@ __global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
     const int wend = start + ((end - start + 7) & ~7);   // written slots
 
     const u32 row_bytes = (u32)args.k * BITS / 8;
-    const __amdgpu_buffer_rsrc_t rl = make_rsrc(args.lhs, args.lhs_bytes);
-    const __amdgpu_buffer_rsrc_t rr = make_rsrc(args.rhs, args.rhs_bytes);
+    const long long bz = blockIdx.y;
+    const __amdgpu_buffer_rsrc_t rl = make_rsrc(args.lhs + bz * args.lhs_stride, args.lhs_bytes);
+    const __amdgpu_buffer_rsrc_t rr = make_rsrc(args.rhs + bz * args.rhs_stride, args.rhs_bytes);
+    int *const out = args.out + bz * args.out_stride;
 
     // K runs in passes of up to NKC 64-deep chunks; every pass holds its A
     // operands in registers and adds its partial sums (exact mod 2^32, the
@@ -276,14 +284,14 @@ __global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
                                 const size_t o =
                                     OUT_REF ? (size_t)(j >> 3) * 8 * V + 8 * v + (j & 7) : (size_t)j * V + v;
                                 // later passes add to what this lane stored before
-                                args.out[o] = ps == 0 ? val : (int)((u32)args.out[o] + (u32)val);
+                                out[o] = ps == 0 ? val : (int)((u32)out[o] + (u32)val);
                             }
                         };
                         if constexpr (!OUT_REF && HOME == 4) {
-                            if ((reinterpret_cast<uintptr_t>(args.out) & 15u) == 0) {
+                            if ((reinterpret_cast<uintptr_t>(out) & 15u) == 0) {
                                 // [slot][V]: the lane's rows 4g..4g+3 are one 16-byte run
                                 v4i val = j < end ? v4i{res[0], res[1], res[2], res[3]} : v4i{0, 0, 0, 0};
-                                v4i *o = reinterpret_cast<v4i *>(args.out + (size_t)j * V + 4 * g);
+                                v4i *o = reinterpret_cast<v4i *>(out + (size_t)j * V + 4 * g);
                                 if (ps != 0) {
                                     const v4i prev = *o;
 #pragma unroll

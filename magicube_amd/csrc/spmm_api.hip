@@ -1,23 +1,12 @@
-// C ABI of the SpMM family: argument validation, tile-config table and
-// dispatch to the per-pair instantiation units.
-#include <map>
-#include <mutex>
-#include <tuple>
-
+// C ABI of the SpMM family: argument validation, the tile-config table and
+// dispatch to the per-pair instantiation units.  The entry points keep no
+// state: the tile is the default for the shape unless the call names one.
 #include "spmm_dispatch.hpp"
 
 namespace mc {
 namespace {
 
-std::mutex g_cfg_mutex;
-std::map<std::tuple<int, int, int, int>, int> g_cfg_table;  // (lhs, rhs, V, n) -> config
-
-int default_config(int lhs_bits, int rhs_bits, int V, int n, int m_vec) {
-    {
-        std::lock_guard<std::mutex> lk(g_cfg_mutex);
-        auto it = g_cfg_table.find(std::make_tuple(lhs_bits, rhs_bits, V, n));
-        if (it != g_cfg_table.end()) return it->second;
-    }
+int default_config(int rhs_bits, int n, long long m_vec) {
     // One dword of B per lane and one k-block of gathers in flight per wave:
     // latency is hidden by more resident waves rather than deeper per-wave
     // pipelines.  Up to four waves share a row (one per 16 * 32/bits
@@ -29,7 +18,7 @@ int default_config(int lhs_bits, int rhs_bits, int V, int n, int m_vec) {
     const int tile_cols = 16 * (32 / rhs_bits);
     const int tiles = (n + tile_cols - 1) / tile_cols;
     const int tpb = tiles >= 4 ? 4 : (tiles >= 2 ? 2 : 1);
-    const long long waves = (long long)m_vec * ((tiles + tpb - 1) / tpb) * tpb;
+    const long long waves = m_vec * ((tiles + tpb - 1) / tpb) * tpb;
     if (waves < 2048) return tpb == 4 ? 20 : 22;
     return tpb == 4 ? 17 : (tpb == 2 ? 16 : 15);
 }
@@ -54,8 +43,11 @@ bool pair_supported(int lhs_bits, int rhs_bits) {
 
 mc_status spmm_entry(int lhs_bits, int rhs_bits, mc_signedness sign, int config, int m_vec, int vec_length, int n,
                      int k, const int *row_indices, const int *row_offsets, const int *column_indices,
-                     const int *values, const int *rhs_matrix, int *output_matrix, hipStream_t stream) {
+                     const int *values, const int *rhs_matrix, int *output_matrix, hipStream_t stream,
+                     int batch = 1, long long values_stride = 0, long long rhs_stride = 0,
+                     long long out_stride = 0) {
     if (!pair_supported(lhs_bits, rhs_bits)) return MC_EUNSUPPORTED;
+    if (batch < 1 || batch > 65535 || values_stride < 0 || rhs_stride < 0 || out_stride < 0) return MC_EINVAL_SHAPE;
     if (sign != MC_SIGNED && sign != MC_UNSIGNED) return MC_EINVAL_ARG;
     if (!valid_v(vec_length)) return MC_EINVAL_V;
     if (m_vec < 0 || n <= 0 || k <= 0) return MC_EINVAL_SHAPE;
@@ -67,8 +59,10 @@ mc_status spmm_entry(int lhs_bits, int rhs_bits, mc_signedness sign, int config,
     if (m_vec == 0) return MC_OK;
     if (!row_offsets || !column_indices || !values || !rhs_matrix || !output_matrix) return MC_EINVAL_ARG;
     if (config < -1 || config >= kNumSpmmConfigs) return MC_EINVAL_ARG;
-    int cfg = config >= 0 ? config : default_config(lhs_bits, rhs_bits, vec_length, n, m_vec);
+    // a batched launch fills the GPU with (row, tile) x batch workgroups
+    int cfg = config >= 0 ? config : default_config(rhs_bits, n, (long long)m_vec * batch);
     if (!config_supported(cfg, rhs_bits)) return MC_EUNSUPPORTED;
+    if (batch > 1 && kSpmmConfigs[cfg].kind == 1) return MC_EUNSUPPORTED;   // B-stationary: one matrix
     if (kSpmmConfigs[cfg].kind == 1 && !stationary_fits(cfg, n, k)) {
         if (config >= 0) return MC_EINVAL_SHAPE;
         cfg = 7;   // the gather kernel takes any k
@@ -89,6 +83,10 @@ mc_status spmm_entry(int lhs_bits, int rhs_bits, mc_signedness sign, int config,
     a.rhs = reinterpret_cast<const unsigned char *>(rhs_matrix);
     a.out = output_matrix;
     a.rhs_bytes = (unsigned)rhs_bytes;
+    a.values_stride = values_stride;
+    a.rhs_stride = rhs_stride;
+    a.out_stride = out_stride;
+    a.batch = batch;
     const bool sgn = sign == MC_SIGNED;
     if (lhs_bits == 8 && rhs_bits == 8) return spmm_launch_l8r8(sgn, vec_length, cfg, a, stream);
     if (lhs_bits == 16 && rhs_bits == 8) return spmm_launch_l16r8(sgn, vec_length, cfg, a, stream);
@@ -176,16 +174,14 @@ mc_status mc_spmm_run(int lhs_bits, int rhs_bits, mc_signedness sign, int config
                           column_indices, values, rhs_matrix, output_matrix, stream);
 }
 
-mc_status mc_spmm_set_tile_config(int lhs_bits, int rhs_bits, int vec_length, int n, int config) {
-    if (!mc::pair_supported(lhs_bits, rhs_bits)) return MC_EUNSUPPORTED;
-    if (!mc::valid_v(vec_length)) return MC_EINVAL_V;
-    if (config < -1 || config >= mc::kNumSpmmConfigs) return MC_EINVAL_ARG;
-    if (config >= 0 && !mc::config_supported(config, rhs_bits)) return MC_EUNSUPPORTED;
-    std::lock_guard<std::mutex> lk(mc::g_cfg_mutex);
-    auto key = std::make_tuple(lhs_bits, rhs_bits, vec_length, n);
-    if (config < 0) mc::g_cfg_table.erase(key);
-    else mc::g_cfg_table[key] = config;
-    return MC_OK;
+mc_status mc_spmm_batched(int lhs_bits, int rhs_bits, mc_signedness sign, int config, int batch, int m_vec,
+                          int vec_length, int n, int k, const int *row_indices, const int *row_offsets,
+                          const int *column_indices, const int *values, long long values_stride_bytes,
+                          const int *rhs_matrix, long long rhs_stride_bytes, int *output_matrix,
+                          long long out_stride, hipStream_t stream) {
+    return mc::spmm_entry(lhs_bits, rhs_bits, sign, config, m_vec, vec_length, n, k, row_indices, row_offsets,
+                          column_indices, values, rhs_matrix, output_matrix, stream, batch, values_stride_bytes,
+                          rhs_stride_bytes, out_stride);
 }
 
 }  // extern "C"

@@ -46,7 +46,7 @@ mc_status spmm_launch_one(SpmmArgs a, hipStream_t stream) {
     const long long blocks = (long long)a.m_vec * groups;
     if (blocks <= 0) return MC_OK;
     if (blocks > 0x7fffffffLL) return MC_EINVAL_SHAPE;
-    hipLaunchKernelGGL((spmm_kernel<V, AB, BB, SGN, LW, TPB, SK, D, WPS>), dim3((unsigned)blocks),
+    hipLaunchKernelGGL((spmm_kernel<V, AB, BB, SGN, LW, TPB, SK, D, WPS>), dim3((unsigned)blocks, (unsigned)a.batch),
                        dim3(64 * TPB * SK), 0, stream, a);
     return check_launch();
 }

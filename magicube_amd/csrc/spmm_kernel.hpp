@@ -53,6 +53,14 @@ struct SpmmArgs {
     int *out;
     int tiles_n;                // ceil(n / TILE_N)
     unsigned rhs_bytes;         // k * n * BB / 8  (< 2^31)
+    // batched launch (blockIdx.y = matrix b of one shared mask, gather
+    // kernel only): matrix b's values / rhs start values_stride /
+    // rhs_stride bytes after matrix b-1's, its output out_stride int32
+    // after; 0 for a single matrix
+    long long values_stride;
+    long long rhs_stride;
+    long long out_stride;
+    int batch;                  // matrices in the launch (gridDim.y), >= 1
 };
 
 template <int AB, int BB, bool SGN> struct Pair;
@@ -306,8 +314,10 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) __attribute__((amdgpu_waves_per_
     // lanes past the last column add kOobAdd: real rows then read out of range
     const u32 coff = col_ok ? (u32)col_dw * 4u : kOobAdd;
 
-    const __amdgpu_buffer_rsrc_t rb = make_rsrc(args.rhs, args.rhs_bytes);
-    const __amdgpu_buffer_rsrc_t ra = make_rsrc(args.values, 0x7FFFFFFFu);
+    const long long bz = blockIdx.y;
+    const __amdgpu_buffer_rsrc_t rb = make_rsrc(args.rhs + bz * args.rhs_stride, args.rhs_bytes);
+    const __amdgpu_buffer_rsrc_t ra = make_rsrc(args.values + bz * args.values_stride, 0x7FFFFFFFu);
+    int *const out = args.out + bz * args.out_stride;
 
     // plane / vector lane of this lane's A row in each row tile
     int a_p[RT], a_v[RT];
@@ -540,7 +550,7 @@ __global__ __launch_bounds__(64 * TPB * SPLITK) __attribute__((amdgpu_waves_per_
 #pragma unroll
     for (int i = 0; i < HOME; ++i) {
         const int v = 4 * g + i;
-        int *orow = args.out + (size_t)(m * V + v) * (size_t)args.n + (size_t)col_dw * CPD;
+        int *orow = out + (size_t)(m * V + v) * (size_t)args.n + (size_t)col_dw * CPD;
         if constexpr (NCOL % 4 == 0) {
 #pragma unroll
             for (int c4 = 0; c4 < NCOL; c4 += 4) {

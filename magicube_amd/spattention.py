@@ -33,8 +33,16 @@ four pairs (spattention.py:194-215):
 * 4, 4: as 8, 4 with P clamped at 7 (the reference's 4-bit P,
   q_softmax_kernel.cu:84-118).
 
-``need_weights=True`` and ``key_padding_mask`` are rejected; the reference's
-encoder uses neither (end_to_end.py:70-73).
+``key_padding_mask`` ([batch, L], True or nonzero = padded key) removes the
+padded key columns from each sample's mask, as ``torch.nn.MultiheadAttention``
+does; a row whose every key is padded gets 0.  The reference checks only the
+mask's shape (spattention.py:171-181) and its encoder passes none
+(end_to_end.py:70-73).  ``need_weights=True`` also returns the head-averaged
+weights [batch, L, L]: the layer's quantized probabilities P / 32 at the
+mask's nonzeros, 0 elsewhere (from the fused kernel's P output).  The
+reference's own ``need_weights`` branch views its packed P as a dense
+[batch, heads, L, L] tensor (spattention.py:225-228), which fails for any
+sparse mask; its encoder calls with ``need_weights=False``.
 
 ``SparseTransformerEncoder`` / ``TransformerEncoder`` restate the encoder
 benchmark models (end_to_end.py:47-183): embedding, sinusoidal positions,
@@ -44,6 +52,7 @@ Linear -> residual, then LayerNorm and a Linear head.
 from __future__ import annotations
 
 import math
+from collections import OrderedDict
 
 import numpy as np
 import torch
@@ -53,6 +62,10 @@ from torch import nn
 from . import formats
 from .attention import (HEAD_DIM, P_REFERENCE, P_REFERENCE_4BIT, ROUND_TRUNC, AttentionBatch, AttentionMatrix,
                         ResidentMask)
+
+# plans (buffers, quantized operands, descriptors) kept per module, least
+# recently used dropped first
+MAX_PLANS = 8
 
 SCALE_QKV = 36.0    # spattention.py:51
 SCALE_SFMX = 32.0   # spattention.py:53
@@ -102,31 +115,96 @@ def mask_from_reference(row_indices: torch.Tensor, row_offsets: torch.Tensor, co
                         row_nnz=(ro[1::2] - ro[0::2]).contiguous(), pairs=pairs)
 
 
+def _real_slots(m: ResidentMask):
+    """(slot, vector row) of every real nonzero of a mask, in row order."""
+    pairs = m.pairs if m.pairs is not None else m.row_offsets.detach().to("cpu").numpy()
+    starts, ends = pairs[0::2].astype(np.int64), pairs[1::2].astype(np.int64)
+    counts = ends - starts
+    rows = np.repeat(np.arange(m.m_vec), counts)
+    slots = np.repeat(starts - np.concatenate([[0], np.cumsum(counts)[:-1]]), counts) + np.arange(counts.sum())
+    return slots, rows
+
+
+def _padded_mask(base: ResidentMask, base_cols: np.ndarray, drop: np.ndarray, device) -> ResidentMask:
+    """``base`` without the key columns where ``drop`` is True, rows
+    re-aligned to 16 slots."""
+    slots, rows = _real_slots(base)
+    cols = base_cols[slots]
+    keep = ~drop[cols]
+    ro = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=base.m_vec))]).astype(np.int32)
+    csr = formats.VectorCSR(base.m_vec, base.L, base.V, ro, cols[keep].astype(np.int32))
+    return ResidentMask.from_csr(csr, device)
+
+
 class _Plan:
-    """Per (shape, mask, precision) state of SpMultiheadAttention.
+    """Per (shape, mask, padding, precision) state of SpMultiheadAttention.
 
     The in-projection writes ``qkv`` ([L][batch][3 * E] fp16); the quantize
     launch reads each (sample, head)'s [L][64] slices of it in place, and the
     fused kernel writes each head's output as fp16 straight into ``o``
     ([L][batch][E], the output projection's input), so no permute copies run
-    between the projections and the attention kernels."""
+    between the projections and the attention kernels.  ``masks`` holds one
+    mask per sample (the same one without a padding mask).  ``keep`` holds
+    the caller's tensors the plan was keyed on, so their addresses cannot be
+    handed to other tensors while the plan exists."""
 
-    def __init__(self, L: int, bsz: int, heads: int, mask: ResidentMask, lhs_pre: int, rhs_pre: int, device):
+    def __init__(self, L: int, bsz: int, heads: int, masks: list, lhs_pre: int, rhs_pre: int, device, keep=()):
         E = heads * HEAD_DIM
+        self.L, self.bsz, self.heads, self.masks, self.keep = L, bsz, heads, masks, tuple(keep)
+        self.lhs_pre, self.rhs_pre, self.device = lhs_pre, rhs_pre, device
         self.qkv = torch.empty((L, bsz, 3 * E), dtype=torch.float16, device=device)
         self.o = torch.empty((L, bsz, E), dtype=torch.float16, device=device)
+        self.batch = self._batch(debug=False)
+        self.debug_batch = None
+        self.weight_index = None
+
+    def _batch(self, debug: bool) -> AttentionBatch:
+        L, bsz, heads, E = self.L, self.bsz, self.heads, self.heads * HEAD_DIM
         q5 = self.qkv.view(L, bsz, 3, heads, HEAD_DIM)
         o4 = self.o.view(L, bsz, heads, HEAD_DIM)
         mats, outs = [], []
         for b in range(bsz):
             for h in range(heads):
-                mats.append(AttentionMatrix(q5[:, b, 0, h], q5[:, b, 1, h], q5[:, b, 2, h], mask))
+                mats.append(AttentionMatrix(q5[:, b, 0, h], q5[:, b, 1, h], q5[:, b, 2, h], self.masks[b]))
                 outs.append(o4[:, b, h])
         n_mat = bsz * heads
-        self.batch = AttentionBatch(mats, qk_bits=rhs_pre, V=mask.V,
-                                    p_mode=P_REFERENCE_4BIT if lhs_pre == 4 else P_REFERENCE, p_scale=SCALE_SFMX,
-                                    scales=[(SCALE_QKV, SCALE_QKV, SCALE_QKV)] * n_mat, rounding=ROUND_TRUNC,
-                                    device=device, outputs=outs, v_range4=rhs_pre == 4)
+        return AttentionBatch(mats, qk_bits=self.rhs_pre, V=self.masks[0].V,
+                              p_mode=P_REFERENCE_4BIT if self.lhs_pre == 4 else P_REFERENCE, p_scale=SCALE_SFMX,
+                              scales=[(SCALE_QKV, SCALE_QKV, SCALE_QKV)] * n_mat, rounding=ROUND_TRUNC,
+                              device=self.device, outputs=outs, v_range4=self.rhs_pre == 4, debug=debug)
+
+    def forward_with_weights(self) -> torch.Tensor:
+        """Runs the layer through the kernel that also writes P and returns
+        the head-averaged weights P / 32 as [batch, L, L] (float32)."""
+        if self.debug_batch is None:
+            self.debug_batch = self._batch(debug=True)
+            self.weight_index = []
+            for m in self.masks:
+                # every real (slot, v) of the mask: its byte in the P layout
+                # [(slot / 16)][v][16] and its (row, column) in the dense weights
+                slots, rows = _real_slots(m)
+                V = m.V
+                v = np.arange(V)
+                p_idx = ((slots[:, None] // 16) * 16 * V + 16 * v[None, :] + slots[:, None] % 16).ravel()
+                r_idx = (rows[:, None] * V + v[None, :]).ravel()
+                t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.device)  # noqa: E731
+                c_idx = m.col_indices[t(slots)].long().repeat_interleave(V)
+                self.weight_index.append((t(p_idx), t(r_idx), c_idx))
+        self.debug_batch.forward()
+        w = torch.zeros((self.bsz, self.L, self.L), dtype=torch.float32, device=self.device)
+        mats = self.debug_batch.mats
+        for b, (p_idx, r_idx, cols) in enumerate(self.weight_index):
+            acc = torch.zeros(p_idx.numel(), dtype=torch.float32, device=self.device)
+            for h in range(self.heads):
+                acc += mats[b * self.heads + h].p_out[p_idx].to(torch.int8).float()
+            w[b].index_put_((r_idx, cols), acc / (SCALE_SFMX * self.heads))
+        return w
+
+
+def _key_tensors(*ts):
+    """Identity of the caller's mask tensors: address, size and version
+    counter (in-place writes bump it)."""
+    return tuple((t.data_ptr(), t.numel(), t._version) if t is not None else None for t in ts)
 
 
 class SpMultiheadAttention(nn.MultiheadAttention):
@@ -140,14 +218,14 @@ class SpMultiheadAttention(nn.MultiheadAttention):
             raise ValueError(f"head dimension must be {HEAD_DIM}")
         if add_bias_kv or add_zero_attn:
             raise ValueError("bias_kv and zero attention are not supported")
-        self._plans: dict = {}
+        self._plans: OrderedDict = OrderedDict()
+
+    def clear_plans(self) -> None:
+        """Drops every cached plan (and the device memory it holds)."""
+        self._plans.clear()
 
     def forward(self, query, key, value, key_padding_mask=None, need_weights=True, row_indices=None,
                 row_offsets=None, column_indices=None, vec_length=2, lhs_pre=8, rhs_pre=8, mask=None):
-        if need_weights:
-            raise NotImplementedError("attention weights are not materialized: call with need_weights=False")
-        if key_padding_mask is not None:
-            raise NotImplementedError("key_padding_mask is not supported")
         if (lhs_pre, rhs_pre) not in ((8, 8), (16, 8), (8, 4), (4, 4)):
             raise NotImplementedError(f"precisions lhs_pre={lhs_pre}, rhs_pre={rhs_pre}: the reference's pairs "
                                       "are (8, 8), (16, 8), (8, 4) and (4, 4)")
@@ -156,20 +234,43 @@ class SpMultiheadAttention(nn.MultiheadAttention):
         H = self.num_heads
         if key.shape != query.shape or value.shape != query.shape:
             raise ValueError("query, key and value must have the same shape")
+        if key_padding_mask is not None and tuple(key_padding_mask.shape) != (bsz, L):
+            raise ValueError(f"key_padding_mask must be [batch, L] = [{bsz}, {L}]")
         if mask is None:
             if row_offsets is None or column_indices is None:
                 raise ValueError("a mask (row_offsets, column_indices) is required")
-            key_ = (L, bsz, p_bits, rhs_pre, vec_length, row_offsets.data_ptr(), column_indices.data_ptr())
+            key_ = (L, bsz, p_bits, rhs_pre, vec_length) + _key_tensors(row_offsets, column_indices, row_indices)
         else:
             key_ = (L, bsz, p_bits, rhs_pre, mask.V, id(mask))
+        if key_padding_mask is not None:
+            key_ += _key_tensors(key_padding_mask)
         plan = self._plans.get(key_)
         if plan is None:
+            keep = [t for t in (row_offsets, column_indices, row_indices, key_padding_mask) if t is not None]
             if mask is None:
                 ri = row_indices if row_indices is not None else torch.zeros(1, dtype=torch.int32,
                                                                                device=query.device)
                 mask = mask_from_reference(ri, row_offsets, column_indices, L, vec_length)
-            plan = _Plan(L, bsz, H, mask, p_bits, rhs_pre, query.device)
+            else:
+                keep.append(mask)
+            if key_padding_mask is None:
+                masks = [mask] * bsz
+            else:
+                drop = key_padding_mask.detach().to("cpu").numpy().astype(bool)
+                base_cols = mask.col_indices.detach().to("cpu").numpy()
+                uniq = {}
+                masks = []
+                for b in range(bsz):
+                    k = drop[b].tobytes()
+                    if k not in uniq:
+                        uniq[k] = _padded_mask(mask, base_cols, drop[b], query.device)
+                    masks.append(uniq[k])
+            plan = _Plan(L, bsz, H, masks, p_bits, rhs_pre, query.device, keep)
             self._plans[key_] = plan
+            while len(self._plans) > MAX_PLANS:
+                self._plans.popitem(last=False)
+        else:
+            self._plans.move_to_end(key_)
 
         # in-projection into the plan's [L][batch][3E] buffer, read in place
         # by the quantize launch
@@ -187,8 +288,12 @@ class SpMultiheadAttention(nn.MultiheadAttention):
             for i, x in enumerate((query, key, value)):
                 part = F.linear(x.reshape(L * bsz, E), w[i * E:(i + 1) * E], None if b is None else b[i * E:(i + 1) * E])
                 qkv2[:, i * E:(i + 1) * E].copy_(part)
-        plan.batch.forward()
-        return F.linear(plan.o, self.out_proj.weight, self.out_proj.bias), None
+        if need_weights:
+            weights = plan.forward_with_weights().to(query.dtype)
+        else:
+            plan.batch.forward()
+            weights = None
+        return F.linear(plan.o, self.out_proj.weight, self.out_proj.bias), weights
 
 
 class PositionalEncoding(nn.Module):

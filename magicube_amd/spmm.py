@@ -10,7 +10,7 @@ the reference returns a non-success ``cudaError_t``.  12-bit lanes sit in
 from __future__ import annotations
 
 import ctypes
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 import torch
@@ -129,11 +129,6 @@ def tile_configs():
     return out
 
 
-def set_tile_config(lhs_bits: int, rhs_bits: int, vec_length: int, n: int, config: int) -> None:
-    _lib.check(_lib.lib().mc_spmm_set_tile_config(lhs_bits, rhs_bits, vec_length, n, config),
-               "mc_spmm_set_tile_config")
-
-
 @dataclass
 class DeviceSpmm:
     """A prepared sparse operand resident in HBM, ready for repeated products."""
@@ -143,6 +138,9 @@ class DeviceSpmm:
     col_indices: torch.Tensor
     values: torch.Tensor
     row_indices: torch.Tensor
+    # tile config for products with a given n (the autotuner's choice;
+    # absent: the library's default for the shape)
+    configs: dict = field(default_factory=dict)
 
     @classmethod
     def upload(cls, prep: PreparedSpmm, device="cuda") -> "DeviceSpmm":
@@ -166,6 +164,8 @@ class DeviceSpmm:
         p = self.prep
         if out is None:
             out = torch.empty((p.m_vec * p.V, n), dtype=torch.int32, device=rhs.device)
+        if config is None:
+            config = self.configs.get(n)
         wmma_spmm(p.lhs_bits, p.rhs_bits, p.m_vec, p.V, n, p.k, self.row_indices, self.row_offsets,
                   self.col_indices, self.values, rhs, out, sign, config, stream)
         return out

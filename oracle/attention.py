@@ -103,3 +103,107 @@ def error_vs_fp64(o: np.ndarray, o_ref: np.ndarray) -> dict:
     den = np.linalg.norm(o_ref)
     return {"max_abs": float(np.abs(diff).max()) if diff.size else 0.0,
             "rel_fro": float(np.linalg.norm(diff) / den) if den > 0 else 0.0}
+
+
+# ---------------------------------------------------------------------------
+# The reference's per-op numerics (the sptrans ops of
+# end2end_eval/sparse_transformer_magicube/src/cuda).  Masks are the aligned
+# vector-CSR: pairs = 2 * m_vec (start, end), cols = aligned column indices.
+# ---------------------------------------------------------------------------
+
+def ref_bsddmm(lhs: np.ndarray, rhs: np.ndarray, pairs, cols, V: int, scale: float) -> np.ndarray:
+    """fp16 [batch, nnz * V]: half(float32(S) / float32(scale)) with S the
+    exact int sum over k of lhs[b, i V + v, k] * rhs[b, col_j, k] at every real
+    slot j of vector row i, vector-major [slot][v]; padding slots 0
+    (deq_sddmm_kernel.cu:565-636, sddmm_utils/output_tile.h:37-43).
+    lhs [batch, m, k] and rhs [batch, n, k] hold integer lanes."""
+    batch = lhs.shape[0]
+    nnz = len(cols)
+    out = np.zeros((batch, nnz, V), dtype=np.float16)
+    for i in range(len(pairs) // 2):
+        s0, e0 = int(pairs[2 * i]), int(pairs[2 * i + 1])
+        if e0 <= s0:
+            continue
+        c = np.asarray(cols[s0:e0], dtype=np.int64)
+        for b in range(batch):
+            S = lhs[b, i * V:(i + 1) * V].astype(np.int64) @ rhs[b, c].astype(np.int64).T     # [V, n]
+            out[b, s0:e0] = (S.T.astype(np.float32) / np.float32(scale)).astype(np.float16)
+    return out.reshape(batch, nnz * V)
+
+
+def ref_softmax_f16(values: np.ndarray, pairs, rows, V: int, sqrt_dk: float, scale: float, bits: int) -> np.ndarray:
+    """int P [batch, nnz, V] of the reference's quantized softmax
+    (q_softmax_kernel.cu:180-271 with vsoftmax_16b/_8b/_4b :84-177):
+    s = float32(half) * sqrt_dk; the running max of row v starts at -1e10
+    for v = 0 and at 0 for v >= 1 (the register array is initialised as
+    {-1e+10}, :209); P = trunc(clamp(exp(s - max) / sum * scale)) in the
+    width's range; padding slots 0.  The sum runs in numpy's order (the
+    reference reduces with CUB), so P may differ by one step from any
+    fixed-order evaluation."""
+    lo, hi = {16: (-32768.0, 32767.0), 8: (-128.0, 127.0), 4: (-8.0, 7.0)}[bits]
+    batch = values.shape[0]
+    nnz = values.shape[1] // V
+    vals = values.reshape(batch, nnz, V)
+    P = np.zeros((batch, nnz, V), dtype=np.int64)
+    init = np.array([-1e10] + [0.0] * (V - 1), dtype=np.float32)
+    for m in rows:
+        s0, e0 = int(pairs[2 * m]), int(pairs[2 * m + 1])
+        if e0 <= s0:
+            continue
+        for b in range(batch):
+            s = vals[b, s0:e0].astype(np.float32) * np.float32(sqrt_dk)           # [nz, V]
+            mx = np.maximum(s.max(axis=0), init)
+            e = np.exp(s - mx).astype(np.float32)
+            sm = e.sum(axis=0, dtype=np.float32)
+            p = (e / sm).astype(np.float32) * np.float32(scale)
+            P[b, s0:e0] = np.trunc(np.clip(p, lo, hi)).astype(np.int64)
+    return P
+
+
+def pack_p_reference(P: np.ndarray, V: int, bits: int) -> np.ndarray:
+    """[nnz, V] P -> the reference's packed bytes (nnz * V * bits / 8):
+    8-bit [(i/16)][v][16]; 16-bit [(i/16)][v][low 16][high 16]; 4-bit
+    [(i/32)][v][16] with byte j = slot 2j (low nibble) | 2j+1 (high)
+    (q_softmax_kernel.cu:253-268).  nnz must be a multiple of 16 (32 for
+    4-bit)."""
+    nnz = P.shape[0]
+    p = P.astype(np.int64)
+    if bits == 8:
+        return (p.reshape(nnz // 16, 16, V).transpose(0, 2, 1) & 255).astype(np.uint8).ravel()
+    if bits == 16:
+        g = p.reshape(nnz // 16, 16, V).transpose(0, 2, 1) & 0xFFFF          # [G, V, 16]
+        return np.stack([g & 255, g >> 8], axis=2).astype(np.uint8).ravel()   # [G, V, 2, 16]
+    g = p.reshape(nnz // 32, 32, V).transpose(0, 2, 1) & 15                  # [G, V, 32]
+    return (g[..., 0::2] | (g[..., 1::2] << 4)).astype(np.uint8).ravel()
+
+
+def unpack_p_reference(raw: np.ndarray, nnz: int, V: int, bits: int) -> np.ndarray:
+    """Inverse of :func:`pack_p_reference` (signed values)."""
+    b = np.asarray(raw, dtype=np.uint8)
+    if bits == 8:
+        g = b[: nnz * V].reshape(nnz // 16, V, 16).astype(np.int8).astype(np.int64)
+        return g.transpose(0, 2, 1).reshape(nnz, V)
+    if bits == 16:
+        g = b[: nnz * V * 2].reshape(nnz // 16, V, 2, 16).astype(np.int64)
+        w = (g[:, :, 0] | (g[:, :, 1] << 8)).astype(np.uint16).astype(np.int16).astype(np.int64)
+        return w.transpose(0, 2, 1).reshape(nnz, V)
+    g = b[: nnz * V // 2].reshape(nnz // 32, V, 16).astype(np.int64)
+    nib = np.stack([g & 15, g >> 4], axis=3).reshape(nnz // 32, V, 32)
+    nib = np.where(nib > 7, nib - 16, nib)
+    return nib.transpose(0, 2, 1).reshape(nnz, V)
+
+
+def ref_bspmm(P: np.ndarray, v_lanes: np.ndarray, pairs, cols, V: int, scale: float) -> np.ndarray:
+    """fp16 [batch, m_vec * V, n]: half(float32(O) / float32(scale)) with O the
+    exact int sum of P[b, j, v] * v_lanes[b, col_j, :] over the real slots of
+    each vector row (deq_spmm_kernel.cu:645-670,1287-1361)."""
+    batch, _, n = v_lanes.shape
+    m_vec = len(pairs) // 2
+    out = np.zeros((batch, m_vec * V, n), dtype=np.float16)
+    for i in range(m_vec):
+        s0, e0 = int(pairs[2 * i]), int(pairs[2 * i + 1])
+        c = np.asarray(cols[s0:e0], dtype=np.int64)
+        for b in range(batch):
+            O = P[b, s0:e0].astype(np.int64).T @ v_lanes[b, c].astype(np.int64)              # [V, n]
+            out[b, i * V:(i + 1) * V] = (O.astype(np.float32) / np.float32(scale)).astype(np.float16)
+    return out

@@ -5,6 +5,7 @@ import os
 import re
 
 import pytest
+import torch
 
 from magicube_amd import _lib
 
@@ -97,11 +98,10 @@ def test_tile_config_table(mclib):
     assert all(c[0] in (1, 2, 4) for c in cfgs if c[4] == 0)
     assert all(c[0] in (8, 16) for c in cfgs if c[4] == 1)
     stationary = [i for i, c in enumerate(cfgs) if c[4] == 1]
-    assert stationary and mclib.mc_spmm_set_tile_config(8, 4, 8, 512, stationary[0]) == _lib.MC_EUNSUPPORTED
-    assert mclib.mc_spmm_set_tile_config(8, 8, 8, 512, 2) == _lib.MC_OK
-    assert mclib.mc_spmm_set_tile_config(8, 8, 8, 512, -1) == _lib.MC_OK
-    assert mclib.mc_spmm_set_tile_config(8, 8, 5, 512, 0) == _lib.MC_EINVAL_V
-    assert mclib.mc_spmm_set_tile_config(8, 4, 8, 512, 2) == _lib.MC_EUNSUPPORTED
+    assert stationary
+    # the entry points keep no tile state: there is no setter to change what
+    # the named entry points run
+    assert not hasattr(mclib, "mc_spmm_set_tile_config")
 
 
 def test_format_validation(mclib):
@@ -123,3 +123,45 @@ def test_sddmm_validation(mclib):
     assert mclib.mc_sddmm_run(8, 0, 7, 4, 64, 64, None, p, p, p, p, p, 8, None) == _lib.MC_EINVAL_ARG
     for bits in (12, 2, 32):
         assert mclib.mc_sddmm_run(bits, 0, 0, 4, 64, 64, None, p, p, p, p, p, 8, None) == _lib.MC_EUNSUPPORTED
+
+
+def test_batched_and_reference_op_validation(mclib):
+    """Argument checks of the batched SpMM / SDDMM launches and the
+    reference-semantics softmax and dequantization (no launch happens)."""
+    P = ctypes.c_void_p(16)
+    stationary = [i for i, c in enumerate(__import__("magicube_amd.spmm", fromlist=["x"]).tile_configs())
+                  if c[4] == 1]
+    # batch bounds, negative strides, B-stationary configs in a batch
+    for batch in (0, 65536):
+        assert mclib.mc_spmm_batched(8, 8, 0, -1, batch, 4, 8, 64, 64, None, P, P, P, 0, P, 0, P, 0,
+                                     None) == _lib.MC_EINVAL_SHAPE
+        assert mclib.mc_sddmm_batched(8, 0, 1, batch, 4, 64, 64, None, P, P, P, 0, P, 0, P, 0, 8,
+                                      None) == _lib.MC_EINVAL_SHAPE
+    assert mclib.mc_spmm_batched(8, 8, 0, -1, 2, 4, 8, 64, 64, None, P, P, P, -4, P, 0, P, 0,
+                                 None) == _lib.MC_EINVAL_SHAPE
+    assert mclib.mc_spmm_batched(8, 8, 0, stationary[0], 2, 4, 8, 512, 64, None, P, P, P, 0, P, 0, P, 0,
+                                 None) == _lib.MC_EUNSUPPORTED
+    assert mclib.mc_spmm_batched(8, 8, 0, -1, 2, 0, 8, 64, 64, None, P, P, P, 0, P, 0, P, 0, None) == _lib.MC_OK
+    # softmax on fp16 logits: widths, V, batch, alignment of the 16-byte loads
+    assert mclib.mc_attn_softmax_f16(1, 4, 8, None, P, P, 0, 0.125, 32.0, 12, P, 0, None) == _lib.MC_EUNSUPPORTED
+    assert mclib.mc_attn_softmax_f16(1, 4, 3, None, P, P, 0, 0.125, 32.0, 8, P, 0, None) == _lib.MC_EINVAL_V
+    assert mclib.mc_attn_softmax_f16(0, 4, 8, None, P, P, 0, 0.125, 32.0, 8, P, 0, None) == _lib.MC_EINVAL_SHAPE
+    assert mclib.mc_attn_softmax_f16(1, 4, 8, None, P, ctypes.c_void_p(18), 0, 0.125, 32.0, 8, P, 0,
+                                     None) == _lib.MC_EINVAL_ARG
+    assert mclib.mc_attn_softmax_f16(2, 4, 8, None, P, P, 12, 0.125, 32.0, 8, P, 0, None) == _lib.MC_EINVAL_ARG
+    assert mclib.mc_attn_softmax_f16(1, 0, 8, None, P, P, 0, 0.125, 32.0, 8, P, 0, None) == _lib.MC_OK
+    assert mclib.mc_attn_dequant_f16(P, -1, 1.0, P, None) == _lib.MC_EINVAL_SHAPE
+    assert mclib.mc_attn_dequant_f16(None, 4, 1.0, P, None) == _lib.MC_EINVAL_ARG
+    assert mclib.mc_attn_dequant_f16(P, 0, 1.0, P, None) == _lib.MC_OK
+
+
+def test_sptrans_rejects_host_tensors():
+    """The reference-signature ops take device tensors only (no CPU path)."""
+    from magicube_amd import sptrans
+    t = torch.zeros(16, dtype=torch.int32)
+    with pytest.raises(ValueError):
+        sptrans.bsddmm_8b(t, t, t, t.view(1, 4, 4), t.view(1, 4, 4), 2, 8, 1.0)
+    with pytest.raises(ValueError):
+        sptrans.bspmm_8b(t, t, t, t, t.view(1, 4, 4), 2, 8, 8, 1.0)
+    with pytest.raises(ValueError):
+        sptrans.q_bcsr_softmax(t, t, torch.zeros(16, dtype=torch.float16), 0.125, 32.0, 2, 1, 8)

@@ -1,5 +1,5 @@
 """Autotuner on the GPU: every timed config agrees bitwise with the default,
-the winner is installed for the named entry point and cached."""
+the winner is installed on the tuned operand and cached."""
 import numpy as np
 import pytest
 import torch
@@ -20,18 +20,21 @@ def test_tune_spmm_installs_and_caches(bits, tmp_path):
     rhs = torch.from_numpy(formats.pack_lanes(b, bits[1]).view(np.int32)).cuda()
     cache = str(tmp_path / "tune.json")
     print("tune", bits, flush=True)
-    try:
-        res = autotune.tune_spmm(dev, rhs, N, launches=5, reps=3, cache_file=cache)
-        assert not res.cached and res.config in res.timings and len(res.timings) >= 5
-        again = autotune.tune_spmm(dev, rhs, N, cache_file=cache)
-        assert again.cached and again.config == res.config
-        # the named entry point now runs the tuned config; results unchanged
-        out = dev.matmul(rhs, N)
-        ref = dev.matmul(rhs, N, config=res.config)
-        torch.cuda.synchronize()
-        assert torch.equal(out, ref)
-    finally:
-        spmm.set_tile_config(bits[0], bits[1], V, N, -1)
+    res = autotune.tune_spmm(dev, rhs, N, launches=5, reps=3, cache_file=cache)
+    assert not res.cached and res.config in res.timings and len(res.timings) >= 5
+    assert dev.configs[N] == res.config
+    fresh = spmm.DeviceSpmm.upload(dev.prep)
+    again = autotune.tune_spmm(fresh, rhs, N, cache_file=cache)
+    assert again.cached and again.config == res.config and fresh.configs[N] == res.config
+    # the tuned operand runs the tuned config; results unchanged, and an
+    # untuned operand still runs the default
+    out = dev.matmul(rhs, N)
+    ref = dev.matmul(rhs, N, config=res.config)
+    other = spmm.DeviceSpmm.upload(dev.prep)
+    assert other.configs == {}
+    base = other.matmul(rhs, N)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref) and torch.equal(out, base)
 
 
 def test_tune_attention_grid():

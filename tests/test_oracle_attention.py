@@ -140,3 +140,118 @@ def test_4bit_p_and_v_range_modes():
     assert np.array_equal(r2["vq"], np.clip(r0["vq"], -8, 7))
     assert r2["vq"].min() == -8 and r2["vq"].max() == 7       # the clamp is exercised
     assert any((P0 > 7).any() for P0 in r0["P"])
+
+
+# --------------------------------------------------------------------------
+# the reference's per-op numerics (bsddmm, q_bcsr_softmax, bspmm)
+# --------------------------------------------------------------------------
+
+def _aligned_mask(m_vec, n, density, align, seed):
+    """Reference-style aligned mask: (start, end) pairs, -1 padded columns."""
+    rng = np.random.default_rng(seed)
+    pairs, cols, pos = [], [], 0
+    for i in range(m_vec):
+        c = np.sort(rng.choice(n, size=int(rng.binomial(n, density)), replace=False))
+        if i == 1:
+            c = c[:0]                                   # an empty row
+        pairs += [pos, pos + c.size]
+        padded = -(-c.size // align) * align
+        cols += list(c) + [-1] * (padded - c.size)
+        pos += padded
+    return np.array(pairs, np.int32), np.array(cols, np.int32)
+
+
+@pytest.mark.parametrize("bits", [16, 8, 4])
+@pytest.mark.parametrize("V", [2, 4, 8])
+def test_reference_p_layout_round_trip(bits, V):
+    rng = np.random.default_rng(bits * 10 + V)
+    nnz = 64
+    lo, hi = {16: (-32768, 32767), 8: (-128, 127), 4: (-8, 7)}[bits]
+    P = rng.integers(lo, hi + 1, size=(nnz, V))
+    raw = oatt.pack_p_reference(P, V, bits)
+    assert raw.size == nnz * V * bits // 8
+    assert np.array_equal(oatt.unpack_p_reference(raw, nnz, V, bits), P)
+
+
+def test_reference_p_layout_known_positions():
+    # byte offsets computed by hand from q_softmax_kernel.cu:253-268 for V = 2
+    P = np.zeros((32, 2), np.int64)
+    P[3, 1] = 5                                  # 8-bit: (3/16)*2*16 + 16*1 + 3 = 19
+    raw = oatt.pack_p_reference(P, 2, 8)
+    assert raw[19] == 5 and raw.sum() == 5
+    P[:] = 0
+    P[18, 0] = 0x0102                            # 16-bit: (18/16)*2*32 + 0 + 2 = 66, high byte +16
+    raw = oatt.pack_p_reference(P, 2, 16)
+    assert raw[66] == 2 and raw[82] == 1 and raw.sum() == 3
+    P[:] = 0
+    P[5, 1] = 3                                  # 4-bit: byte 16*1 + 5/2 = 18, odd slot -> high nibble
+    P[4, 1] = 6
+    raw = oatt.pack_p_reference(P, 2, 4)
+    assert raw[18] == 0x36 and raw.sum() == 0x36
+
+
+@pytest.mark.parametrize("bits", [16, 8, 4])
+def test_ref_softmax_f16_closed_forms(bits):
+    V, scale = 4, 32.0
+    hi = {16: 32767, 8: 127, 4: 7}[bits]
+    # row 0: one slot -> P = trunc(min(scale, hi)); row 1: three equal logits -> trunc(scale / 3)
+    pairs = np.array([0, 1, 32, 35], np.int32)
+    vals = np.zeros((1, 64 * V), np.float16)
+    vals[0, 32 * V:35 * V] = np.float16(0.75)
+    vals[0, :V] = np.float16(-2.5)
+    P = oatt.ref_softmax_f16(vals, pairs, [0, 1], V, 0.125, scale, bits)
+    assert (P[0, 0] == min(32, hi)).all()
+    assert (P[0, 32:35] == min(10, hi)).all()               # trunc(32 / 3) = 10
+    assert (P[0, 1:32] == 0).all() and (P[0, 35:] == 0).all()
+
+
+def test_ref_softmax_f16_against_float64():
+    V, scale, sqrt_dk = 8, 32.0, 0.125
+    pairs, cols = _aligned_mask(12, 256, 0.2, 16, seed=3)
+    rng = np.random.default_rng(4)
+    vals = (rng.standard_normal((2, cols.size * V)) * 8).astype(np.float16)
+    P = oatt.ref_softmax_f16(vals, pairs, list(range(12)), V, sqrt_dk, scale, 8)
+    worst = 0
+    for i in range(12):
+        s0, e0 = pairs[2 * i], pairs[2 * i + 1]
+        if e0 == s0:
+            continue
+        for b in range(2):
+            s = vals[b].reshape(-1, V)[s0:e0].astype(np.float64) * sqrt_dk
+            p = np.exp(s - s.max(axis=0)) / np.exp(s - s.max(axis=0)).sum(axis=0)
+            worst = max(worst, int(np.abs(P[b, s0:e0] - np.trunc(p * scale)).max()))
+    assert worst <= 1
+    # a dropped 1/sqrt(d) is caught
+    bad = oatt.ref_softmax_f16(vals, pairs, list(range(12)), V, 1.0, scale, 8)
+    assert not np.array_equal(bad, P)
+
+
+def test_ref_bsddmm_and_bspmm_against_dense():
+    V, k, L = 4, 64, 96
+    pairs, cols = _aligned_mask(L // V, L, 0.3, 16, seed=7)
+    rng = np.random.default_rng(8)
+    lhs = rng.integers(-128, 128, size=(2, L, k))
+    rhs = rng.integers(-128, 128, size=(2, L, k))
+    S = oatt.ref_bsddmm(lhs, rhs, pairs, cols, V, 1296.0).reshape(2, -1, V)
+    for b in range(2):
+        dense = lhs[b] @ rhs[b].T                                   # [L, L]
+        for i in range(L // V):
+            s0, e0 = pairs[2 * i], pairs[2 * i + 1]
+            exp = (dense[i * V:(i + 1) * V][:, cols[s0:e0]].T.astype(np.float32) / np.float32(1296)).astype(np.float16)
+            assert np.array_equal(S[b, s0:e0].view(np.uint16), exp.view(np.uint16))
+            pad_end = pairs[2 * i + 2] if i + 1 < L // V else cols.size
+            assert (S[b, e0:pad_end] == 0).all()
+    # bspmm: the sparse P as a dense [L, L] matrix times V
+    P = rng.integers(0, 128, size=(2, cols.size, V))
+    vl = rng.integers(-128, 128, size=(2, L, 64))
+    O = oatt.ref_bspmm(P, vl, pairs, cols, V, 1152.0)
+    for b in range(2):
+        D = np.zeros((L, L), np.int64)
+        for i in range(L // V):
+            s0, e0 = pairs[2 * i], pairs[2 * i + 1]
+            D[i * V:(i + 1) * V, cols[s0:e0]] += P[b, s0:e0].T
+        exp = ((D @ vl[b]).astype(np.float32) / np.float32(1152)).astype(np.float16)
+        assert np.array_equal(O[b].view(np.uint16), exp.view(np.uint16))
+    # a shifted column index is caught
+    bad = oatt.ref_bspmm(P, vl, pairs, np.where(cols >= 0, (cols + 1) % L, -1), V, 1152.0)
+    assert not np.array_equal(bad, O)

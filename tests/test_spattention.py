@@ -65,7 +65,7 @@ def test_plan_reads_projection_in_place_and_writes_heads_into_output():
     E = 64 * H
     cols, _, pairs, order, _ = spattention.static_random_mask_aligned(L // 8, L, 0.9, 16, seed=1, device="cpu")
     mask = spattention.mask_from_reference(order, pairs, cols, L, 8)
-    plan = spattention._Plan(L, bsz, H, mask, 8, 8, "cpu")
+    plan = spattention._Plan(L, bsz, H, [mask] * bsz, 8, 8, "cpu")
     desc = np.frombuffer(plan.batch.desc.numpy().tobytes(), dtype=attention._ATTN_DT)
     qk = np.frombuffer(plan.batch.qk_jobs.numpy().tobytes(), dtype=attention._JOB_DT).reshape(-1, 2)
     vj = np.frombuffer(plan.batch.v_jobs.numpy().tobytes(), dtype=attention._JOB_DT)

@@ -133,3 +133,93 @@ def test_encoder_deterministic_and_graph_replay():
             g.replay()
             s.synchronize()
             assert torch.equal(c, a)
+
+
+def _dense_p_reference(mha, x, kw, L, bsz, heads):
+    """The plain pipeline's int8 P of every (sample, head) as dense [bsz, heads, L, L] / 32."""
+    plan = next(iter(mha._plans.values()))
+    plan.batch.forward_plain()
+    torch.cuda.synchronize()
+    dense = np.zeros((bsz, heads, L, L), np.float64)
+    for n, d in enumerate(plan.batch.mats):
+        p = d.work["p"].cpu().numpy().view(np.int8)
+        pairs, cols = d.pairs, d.col_indices.cpu().numpy()
+        for i in range(d.m_vec):
+            for j in range(pairs[2 * i], pairs[2 * i + 1]):
+                for v in range(8):
+                    dense[n // heads, n % heads, i * 8 + v, cols[j]] = p[(j // 16) * 128 + 16 * v + j % 16] / 32.0
+    return dense
+
+
+def test_need_weights_returns_head_averaged_quantized_probabilities():
+    L, bsz, heads = 256, 2, 2
+    mha, x, kw = _layer(L, bsz, heads, 8, seed=17)
+    print("spattention need_weights", flush=True)
+    with torch.no_grad():
+        out_w, w = mha(x, x, x, need_weights=True, **kw)
+        out, none = mha(x, x, x, need_weights=False, **kw)
+        torch.cuda.synchronize()
+    assert none is None and w.shape == (bsz, L, L) and w.dtype == x.dtype
+    assert torch.equal(out_w, out), "the weights path gives the same output"
+    exp = _dense_p_reference(mha, x, kw, L, bsz, heads).mean(axis=1)
+    got = w.float().cpu().numpy()
+    assert np.allclose(got, exp, atol=2e-3, rtol=0), np.abs(got - exp).max()
+    assert (got >= 0).all() and (got.sum(axis=2) <= 1.0 + 1e-3).all()
+
+
+def test_key_padding_mask_removes_padded_keys():
+    """A padding mask gives each sample the shared mask minus its padded key
+    columns (checked against a hand-filtered mask row by row), padded keys
+    get no weight, the fused output equals the plain pipeline on those
+    masks, and an all-False padding mask changes nothing."""
+    L, bsz, heads = 256, 2, 2
+    mha, x, kw = _layer(L, bsz, heads, 8, seed=19)
+    pad = torch.zeros(bsz, L, dtype=torch.bool, device="cuda")
+    pad[0, 200:] = True
+    pad[1, :32] = True
+    print("spattention key_padding_mask", flush=True)
+    with torch.no_grad():
+        base, _ = mha(x, x, x, need_weights=False, **kw)
+        none_padded, _ = mha(x, x, x, key_padding_mask=torch.zeros_like(pad), need_weights=False, **kw)
+        out, w = mha(x, x, x, key_padding_mask=pad, need_weights=True, **kw)
+        torch.cuda.synchronize()
+    assert torch.equal(base, none_padded)
+    assert not torch.equal(base, out)
+    wn = w.float().cpu().numpy()
+    assert (wn[0, :, 200:] == 0).all() and (wn[1, :, :32] == 0).all()
+    plan = list(mha._plans.values())[-1]
+    pairs, cols = kw["row_offsets"].cpu().numpy(), kw["column_indices"].cpu().numpy()
+    for b in range(bsz):
+        drop = pad[b].cpu().numpy()
+        m = plan.masks[b]
+        mp, mc = m.row_offsets.cpu().numpy(), m.col_indices.cpu().numpy()
+        for i in range(L // 8):
+            c = cols[pairs[2 * i]:pairs[2 * i + 1]]
+            assert np.array_equal(np.sort(mc[mp[2 * i]:mp[2 * i + 1]]), np.sort(c[~drop[c]])), (b, i)
+    with torch.no_grad():
+        fused = plan.o.clone()
+        plain = plan.batch.forward_plain()
+        torch.cuda.synchronize()
+    heads_out = fused.view(L, bsz, heads, 64)
+    for n, o in enumerate(plain):
+        assert np.array_equal(heads_out[:, n // heads, n % heads].cpu().numpy().view(np.uint16),
+                              o.cpu().numpy().astype(np.float16).view(np.uint16)), f"matrix {n}"
+
+
+def test_plan_cache_is_bounded_and_follows_mask_writes():
+    L, bsz, heads = 128, 1, 1
+    mha, x, kw = _layer(L, bsz, heads, 8, seed=23)
+    with torch.no_grad():
+        a, _ = mha(x, x, x, need_weights=False, **kw)
+        # an in-place write to the caller's mask makes a new plan
+        kw["column_indices"][kw["row_offsets"][0]] = (kw["column_indices"][kw["row_offsets"][0]] + 1) % L
+        b, _ = mha(x, x, x, need_weights=False, **kw)
+        torch.cuda.synchronize()
+    assert len(mha._plans) == 2 and not torch.equal(a, b)
+    for s in range(spattention.MAX_PLANS + 3):
+        _, _, kws = _layer(L, bsz, heads, 8, seed=100 + s)
+        with torch.no_grad():
+            mha(x, x, x, need_weights=False, **kws)
+    assert len(mha._plans) == spattention.MAX_PLANS
+    mha.clear_plans()
+    assert len(mha._plans) == 0
