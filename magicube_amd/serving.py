@@ -7,7 +7,11 @@ fp16 Q/K/V and masks are resident in HBM when it arrives, as the output of
 the layers before it would be.  The server
 
 * admits waiting requests first come, first served while the sum of their
-  working sets (quantized Q/K/V and the fp32 outputs) fits an HBM budget;
+  working sets (quantized Q/K/V and the fp32 outputs) fits an HBM budget,
+  every request that fits in one pass; the batched policy builds one
+  resident block (one arena, one vectorized descriptor build) per Q/K width
+  for the whole admitted group, so the host work per request shrinks as
+  groups grow;
 * turns every admitted request that has not started into ONE step: a batched
   quantize launch per operand width and one persistent fused attention
   launch per Q/K width over the union of their (matrix, vector row) items;
@@ -52,10 +56,14 @@ class Request:
     done: float = math.nan
     outputs: list | None = None         # fp32 [L, 64] per matrix (plain policy)
     block: object = None                # attention.ResidentBlock holding the outputs (fused policies)
+    block_range: tuple | None = None    # this request's matrices in a block shared with others
 
     def result(self) -> list:
         """fp32 [L, 64] output of every matrix (valid once the request is done)."""
-        return self.block.outputs() if self.block is not None else self.outputs
+        if self.block is None:
+            return self.outputs
+        outs = self.block.outputs()
+        return outs if self.block_range is None else outs[self.block_range[0]:self.block_range[1]]
 
     @property
     def latency(self) -> float:
@@ -121,15 +129,35 @@ class Server:
         self.peak_used = 0
 
     # -- admission and one step -------------------------------------------
-    def _admit(self, r: Request, now: float) -> None:
-        """Reserves the request's working set; for the fused policies also
-        allocates its arena and builds its descriptors (on the preparation
-        stream, which every step waits for)."""
-        r.admitted = now
-        if self.policy != "plain":
-            with torch.cuda.stream(self.prep_stream):
-                r.block = attention.ResidentBlock(r.matrices, r.scales, r.qk_bits, self.V, self.p_scale, self.device)
-            self.prep_event.record(self.prep_stream)
+    def _admit(self, group: list[Request], now: float) -> None:
+        """Admits a group of requests whose working sets are reserved; for the
+        fused policies also allocates their arenas and builds their
+        descriptors (on the preparation stream, which every step waits for):
+        one block per Q/K width for the whole group under the batched policy
+        (unless steps are capped by ``max_step_items``, which may split a
+        group), one block per request otherwise."""
+        for r in group:
+            r.admitted = now
+        if self.policy == "plain":
+            return
+        with torch.cuda.stream(self.prep_stream):
+            if self.policy == "batched" and self.max_step_items is None:
+                by_width: dict[int, list] = {}
+                for r in group:
+                    by_width.setdefault(r.qk_bits, []).append(r)
+                for bits, rs in by_width.items():
+                    blk = attention.ResidentBlock([m for r in rs for m in r.matrices], [s for r in rs for s in r.scales],
+                                                  bits, self.V, self.p_scale, self.device)
+                    i = 0
+                    for r in rs:
+                        r.block, r.block_range = blk, (i, i + len(r.matrices))
+                        i += len(r.matrices)
+            else:
+                for r in group:
+                    r.block = attention.ResidentBlock(r.matrices, r.scales, r.qk_bits, self.V, self.p_scale,
+                                                      self.device)
+                    r.block_range = None
+        self.prep_event.record(self.prep_stream)
 
     def _launch(self, reqs: list[Request], now: float) -> _Step:
         stream = self.streams[self._next_stream]
@@ -153,8 +181,11 @@ class Server:
             else:
                 stream.wait_event(self.prep_event)
                 blocks: dict[int, list] = {}
-                for r in reqs:
-                    blocks.setdefault(r.qk_bits, []).append(r.block)
+                seen = set()
+                for r in reqs:                      # requests of one group share a block
+                    if id(r.block) not in seen:
+                        seen.add(id(r.block))
+                        blocks.setdefault(r.qk_bits, []).append(r.block)
                 for blks in blocks.values():
                     batch = attention.BlockBatch(blks, p_mode=self.p_mode, p_scale=self.p_scale, device=self.device)
                     batch.forward(stream, self.grid_waves)
@@ -208,13 +239,15 @@ class Server:
             if ready and len(inflight) < self.max_inflight:
                 take, ready = self._take(ready)
                 inflight.append(self._launch(take, now))
-            # admit one request per pass, so a free stream is never kept
-            # waiting behind a long admission burst
-            if waiting and used + waiting[0].ws_bytes <= self.budget:
+            # admit every waiting request that fits, in arrival order
+            group = []
+            while waiting and used + waiting[0].ws_bytes <= self.budget:
                 r = waiting.popleft()
                 used += r.ws_bytes
-                self._admit(r, now)
-                ready.append(r)
+                group.append(r)
+            if group:
+                self._admit(group, now)
+                ready.extend(group)
                 self.peak_used = max(self.peak_used, used)
         span = max(r.done for r in requests) - min(r.arrival for r in requests) if requests else 0.0
         lat = sorted(r.latency for r in requests)
@@ -244,4 +277,4 @@ def reset(requests: list[Request]) -> None:
     """Clears the timing and outputs of a trace so it can be served again."""
     for r in requests:
         r.admitted = r.launched = r.done = math.nan
-        r.outputs = r.block = None
+        r.outputs = r.block = r.block_range = None
