@@ -50,6 +50,8 @@ DRIVERS = {"spmm_benchmark": "spmm_benchmark.cpp", "sddmm_benchmark": "sddmm_ben
 # the reference's vendor-library baseline drivers, on rocSPARSE / hipBLASLt
 VENDOR_DRIVERS = {"baseline_spmm_benchmark": "baseline_spmm_benchmark.cpp",
                   "baseline_sddmm_benchmark": "baseline_sddmm_benchmark.cpp"}
+# measurement programs with device code of their own
+HIP_TOOLS = {"gather_floor": "gather_floor.hip"}
 
 
 def _compile(src: str) -> str:
@@ -98,6 +100,14 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:
             raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    for exe, src in HIP_TOOLS.items():
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        cmd = [HIPCC, *COMMON, *DEVICE_FLAGS, "-x", "hip", path, "-o", os.path.join(HERE, exe)]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"tool build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB

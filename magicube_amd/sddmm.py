@@ -13,7 +13,7 @@ import numpy as np
 import torch
 
 from . import _lib, formats
-from .spmm import _check_bytes, _dptr, _stream
+from .spmm import _check_bytes, _check_device, _dptr, _stream
 
 SIGNED = _lib.MC_SIGNED
 UNSIGNED = _lib.MC_UNSIGNED
@@ -27,6 +27,11 @@ def wmma_sddmm(bits: int, m_vec: int, k: int, n: int, row_indices, row_offsets, 
     if min(m_vec, k, n, vec_length) >= 0:
         _check_bytes(lhs_matrix, m_vec * vec_length * k * bits // 8, "lhs_matrix")
         _check_bytes(rhs_matrix, n * k * bits // 8, "rhs_matrix")
+        # every slot of [start, roundup8(end)) is written; the column-index
+        # array (-1 padded) holds at least as many slots as the rows reach
+        if isinstance(col_indices, torch.Tensor):
+            _check_bytes(output_values, -(-col_indices.numel() // 8) * 8 * vec_length * 4, "output_values")
+    _check_device((row_indices, row_offsets, col_indices, lhs_matrix, rhs_matrix, output_values), "sddmm")
     st = _lib.lib().mc_sddmm_run(bits, sign, layout, m_vec, k, n, _dptr(row_indices), _dptr(row_offsets),
                                  _dptr(col_indices), _dptr(lhs_matrix), _dptr(rhs_matrix), _dptr(output_values),
                                  vec_length, _stream(stream))
@@ -78,7 +83,8 @@ class DeviceMask:
         return cls(csr.m_vec, csr.k, csr.V, an, pairs, cols, t(pairs, 0), t(cols, -1), t(order, 0))
 
     def output_size(self) -> int:
-        return max(self.aligned_num, 8) * self.V
+        """int32 values the launch may write: the padded slot array rounded up to 8, times V."""
+        return -(-max(self.aligned_num, self.col_indices.numel()) // 8) * 8 * self.V
 
     def sddmm(self, bits: int, lhs: torch.Tensor, rhs: torch.Tensor, k: int, out: torch.Tensor | None = None,
               sign: int = SIGNED, layout: int = OUT_REFERENCE, stream=None) -> torch.Tensor:

@@ -54,12 +54,22 @@ def _check_bytes(t, need: int, what: str) -> None:
         raise ValueError(f"{what} holds {t.numel() * t.element_size()} bytes; the launch reads or writes {need}")
 
 
+def _check_device(tensors, what: str) -> None:
+    """Every tensor operand of one launch lives on one device."""
+    devs = {t.device for t in tensors if isinstance(t, torch.Tensor)}
+    if any(d.type != "cuda" for d in devs):
+        raise ValueError(f"{what}: operands must be device tensors")
+    if len(devs) > 1:
+        raise ValueError(f"{what}: operands are on several devices {sorted(map(str, devs))}")
+
+
 def wmma_spmm(lhs_bits: int, rhs_bits: int, m_vec: int, vec_length: int, n: int, k: int, row_indices,
               row_offsets, column_indices, values, rhs_matrix, output_matrix, sign: int = SIGNED,
               config: int | None = None, stream=None) -> None:
     if min(m_vec, vec_length, n, k) >= 0:
         _check_bytes(output_matrix, m_vec * vec_length * n * 4, "output_matrix")
         _check_bytes(rhs_matrix, k * n * rhs_bits // 8, "rhs_matrix")
+    _check_device((row_indices, row_offsets, column_indices, values, rhs_matrix, output_matrix), "spmm")
     lib = _lib.lib()
     args = (m_vec, vec_length, n, k, _dptr(row_indices), _dptr(row_offsets), _dptr(column_indices),
             _dptr(values), _dptr(rhs_matrix), _dptr(output_matrix))

@@ -165,3 +165,13 @@ def test_sptrans_rejects_host_tensors():
         sptrans.bspmm_8b(t, t, t, t, t.view(1, 4, 4), 2, 8, 8, 1.0)
     with pytest.raises(ValueError):
         sptrans.q_bcsr_softmax(t, t, torch.zeros(16, dtype=torch.float16), 0.125, 32.0, 2, 1, 8)
+
+
+def test_spmm_sddmm_mirrors_reject_host_tensors():
+    """The ctypes mirrors never hand a host pointer to a launch."""
+    from magicube_amd import sddmm, spmm
+    t = torch.zeros(64, dtype=torch.int32)
+    with pytest.raises(ValueError):
+        spmm.wmma_spmm(8, 8, 1, 8, 16, 16, None, t, t, t, t, t)
+    with pytest.raises(ValueError):
+        sddmm.wmma_sddmm(8, 1, 64, 16, None, t, t, t, t, t, 8)

@@ -272,3 +272,19 @@ def test_graph_replays_match_eager():
         for e, d in zip(eager, b.mats):
             assert torch.equal(e.view(torch.int32), d.o.view(torch.int32)), f"replay {r}"
         del x
+
+
+def test_bquantization_on_a_side_stream():
+    """Quantizing on another stream (its job descriptor, temporaries and
+    output made on that stream) gives the default-stream result."""
+    rng = np.random.default_rng(11)
+    x = torch.from_numpy((rng.standard_normal((4, 1000, 64)) * 2).astype(np.float16)).cuda()
+    ref = attention.bquantization(x, 8, 36.0)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    for _ in range(3):
+        got = attention.bquantization(x.transpose(0, 1).transpose(0, 1), 8, 36.0, stream=s)
+        s.synchronize()
+        assert torch.equal(got, ref)
+        del got
