@@ -6,19 +6,24 @@ already resident in HBM): L8-R8 SpMM, A 4096 x 4096 vector-sparse with V=8,
 90% sparsity (Poisson nonzero count per 8-row block, seeded), signed
 full-range int8 values, dense B 4096 x 512 int8, int32 output C = A B.
 
-The K timed steps are captured once into a HIP graph on the launch stream
-and replayed; HIP events on that stream bracket the replay.  W untimed steps
-(default 5000, about 45 ms) run first as back-to-back replays of the same
-graph: the GPU reaches its working clocks only under sustained load.  TOPS follow the
-reference's formula 2 * nnz_vec * V * N / t on unpadded nonzeros
-(SpMM/SpMM/spmm_benchmark.cpp:112,637).  The roofline entry uses the
-compulsory bytes nnz_vec*(V + 4) + 4*(m_vec+1) + K*N + 4*M*N against
-MI355X's 8 TB/s HBM3E.  The CPU baseline times the restated host loop
-(compute_ref_integers) on the same arrays, single-threaded and with
-OpenMP, on this machine's host cores.
+The K timed steps are captured once into a HIP graph on the launch stream.
+Untimed replays of it run first: at least W steps and at least
+``--warmup-s`` seconds (default 0.3 s) back to back, as the GPU reaches its
+working clocks only under sustained load.  Five timed replays of exactly K
+steps follow, each between two torch.cuda.synchronize() calls and bracketed
+by HIP events on the launch stream; ``ms_per_step`` is the median replay
+divided by K.  TOPS follow the reference's formula 2 * nnz_vec * V * N / t on
+unpadded nonzeros (SpMM/SpMM/spmm_benchmark.cpp:112,637).  The roofline entry
+uses the compulsory bytes nnz_vec*(V + 4) + 4*(m_vec+1) + K*N + 4*M*N
+against MI355X's 8 TB/s HBM3E.  The CPU baseline times the restated host
+loop (compute_ref_integers) on the same arrays, single-threaded and with
+OpenMP, and on the reference's 512 x 512 CPU case, on this machine's host
+cores.
 
-Prints one JSON line.  ``--gpus N`` with N > 1 exits: the benchmark measures
-one GPU.
+Prints one JSON line (under 2 KB: the headline, its roofline, the CPU
+baseline and a few figures of the other sections); every section's full
+results go to the ``--sections-out`` file.  ``--gpus N`` with N > 1 exits:
+the benchmark measures one GPU.
 """
 from __future__ import annotations
 
@@ -177,9 +182,10 @@ def time_gpu(run_once, steps, warmup, stream, warm_s=0.02, detail=False):
     not between eager launches or host-side setup), and five timed replays
     follow, each bracketed by torch.cuda.synchronize() and by HIP events on
     the capture stream.  Returns the median over the five and the graph; with
-    ``detail`` a dict instead: the first timed replay (exactly ``steps``
-    launches) as ``ms``, the median, the host wall time of that replay and
-    the warm-up actually run."""
+    ``detail`` a dict instead: that median as ``ms`` (every timed replay is
+    exactly ``steps`` launches), the first replay alone (which runs right
+    after the host has waited on the GPU and is 2-3 % slower), the host wall
+    time of the median replay and the warm-up actually run."""
     import torch
     eager = min(warmup, 3)
     for _ in range(eager):
@@ -216,8 +222,10 @@ def time_gpu(run_once, steps, warmup, stream, warm_s=0.02, detail=False):
             times.append(start.elapsed_time(end) / steps)
     if not detail:
         return statistics.median(times), graph
-    return {"ms": times[0], "median_ms": statistics.median(times), "min_ms": min(times), "max_ms": max(times),
-            "host_wall_ms": walls[0] * 1e3, "warmup_steps": eager + done * steps,
+    order = sorted(range(reps), key=lambda i: times[i])
+    mid = order[reps // 2]
+    return {"ms": times[mid], "first_ms": times[0], "min_ms": min(times), "max_ms": max(times),
+            "host_wall_ms": walls[mid] * 1e3, "warmup_steps": eager + done * steps,
             "warmup_s": warm_wall}, graph
 
 
@@ -1134,7 +1142,7 @@ def main():
         "config": {"workload": "spmm_l8r8 M=K=4096 N=512 V=8 s=0.9", "signed": True,
                    "seed": seed, "nnz_vec": nnz, "tile_config": args.config},
         "verified": verified,
-        "timing": {"median_of_5_ms": t["median_ms"], "host_wall_ms": t["host_wall_ms"],
+        "timing": {"replays": 5, "first_ms": t["first_ms"], "host_wall_ms": t["host_wall_ms"],
                    "warmup_steps_run": t["warmup_steps"], "warmup_s": t["warmup_s"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
