@@ -135,7 +135,8 @@ mc_status mc_spmm_run(int lhs_bits, int rhs_bits, mc_signedness sign, int config
  * row_offsets, column_indices): matrix b's prepared values start
  * b * values_stride_bytes after `values`, its B b * rhs_stride_bytes after
  * `rhs_matrix` and its C b * out_stride int32 after `output_matrix`.  One
- * launch, matrices in gridDim.y (batch <= 65535); gather tile configs only
+ * launch, matrices in gridDim.y (batch <= 65535; byte strides multiples of
+ * 4, MC_EINVAL_SHAPE otherwise); gather tile configs only
  * (B-stationary configs return MC_EUNSUPPORTED).  Replaces the batched
  * launchers behind the reference's bspmm_8b / _16b8b / _8b4b / _4b
  * (end2end_eval/sparse_transformer_magicube/src/cuda/deq_spmm.cpp:5-141,
@@ -176,7 +177,8 @@ mc_status mc_sddmm_run(int bits, mc_signedness sign, int layout, int m_vec, int 
                        int vec_length, hipStream_t stream);
 /* Batched SDDMM over `batch` matrices sharing one mask: matrix b's lhs and
  * rhs start b * lhs_stride_bytes / b * rhs_stride_bytes after the first,
- * its output b * out_stride int32 after.  One launch (batch <= 65535).
+ * its output b * out_stride int32 after.  One launch (batch <= 65535);
+ * byte strides are multiples of 4 (MC_EINVAL_SHAPE otherwise).
  * Replaces the batched launcher behind the reference's bsddmm_8b / _4b
  * (end2end_eval/sparse_transformer_magicube/src/cuda/deq_sddmm.cpp:56-93,
  * deq_sddmm_kernel.cu:565-636). */

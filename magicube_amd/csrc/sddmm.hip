@@ -55,6 +55,7 @@ mc_status sddmm_entry(int bits, mc_signedness sign, int layout, int m_vec, int k
                       int *output_values, int vec_length, hipStream_t stream, int batch = 1,
                       long long lhs_stride = 0, long long rhs_stride = 0, long long out_stride = 0) {
     if (batch < 1 || batch > 65535 || lhs_stride < 0 || rhs_stride < 0 || out_stride < 0) return MC_EINVAL_SHAPE;
+    if ((lhs_stride | rhs_stride) & 3) return MC_EINVAL_SHAPE;   // every matrix starts dword aligned
     if (bits != 16 && bits != 8 && bits != 4) return MC_EUNSUPPORTED;
     if (sign != MC_SIGNED && sign != MC_UNSIGNED) return MC_EINVAL_ARG;
     if (layout != MC_SDDMM_OUT_REFERENCE && layout != MC_SDDMM_OUT_VECTOR) return MC_EINVAL_ARG;

@@ -48,6 +48,7 @@ mc_status spmm_entry(int lhs_bits, int rhs_bits, mc_signedness sign, int config,
                      long long out_stride = 0) {
     if (!pair_supported(lhs_bits, rhs_bits)) return MC_EUNSUPPORTED;
     if (batch < 1 || batch > 65535 || values_stride < 0 || rhs_stride < 0 || out_stride < 0) return MC_EINVAL_SHAPE;
+    if ((values_stride | rhs_stride) & 3) return MC_EINVAL_SHAPE;   // every matrix starts dword aligned
     if (sign != MC_SIGNED && sign != MC_UNSIGNED) return MC_EINVAL_ARG;
     if (!valid_v(vec_length)) return MC_EINVAL_V;
     if (m_vec < 0 || n <= 0 || k <= 0) return MC_EINVAL_SHAPE;

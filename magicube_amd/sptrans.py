@@ -244,9 +244,9 @@ def _bspmm(bits_lhs, bits_rhs, row_indices, row_offsets, column_indices, values,
         raise ValueError("rhs_matrix [batch, k, n * bits / 32] expected")
     batch, k, n_int32 = rhs.shape
     n = n_int32 * (32 // bits_rhs)
-    vals = vals.view(batch, -1) if vals.numel() % batch == 0 else None
-    if vals is None:
+    if vals.numel() % batch:
         raise ValueError("values do not split into one P per matrix")
+    vals = vals.view(batch, -1)
     nnz = ci.numel()
     need = nnz * vec_length * bits_lhs // 8
     if vals.shape[1] * 4 < need:

@@ -141,6 +141,12 @@ def test_batched_and_reference_op_validation(mclib):
                                  None) == _lib.MC_EINVAL_SHAPE
     assert mclib.mc_spmm_batched(8, 8, 0, stationary[0], 2, 4, 8, 512, 64, None, P, P, P, 0, P, 0, P, 0,
                                  None) == _lib.MC_EUNSUPPORTED
+    # byte strides that would start a matrix off a dword boundary
+    for vs, rs in ((6, 0), (0, 2)):
+        assert mclib.mc_spmm_batched(8, 8, 0, -1, 2, 4, 8, 64, 64, None, P, P, P, vs, P, rs, P, 0,
+                                     None) == _lib.MC_EINVAL_SHAPE
+        assert mclib.mc_sddmm_batched(8, 0, 1, 2, 4, 64, 64, None, P, P, P, vs, P, rs, P, 0, 8,
+                                      None) == _lib.MC_EINVAL_SHAPE
     assert mclib.mc_spmm_batched(8, 8, 0, -1, 2, 0, 8, 64, 64, None, P, P, P, 0, P, 0, P, 0, None) == _lib.MC_OK
     # softmax on fp16 logits: widths, V, batch, alignment of the 16-byte loads
     assert mclib.mc_attn_softmax_f16(1, 4, 8, None, P, P, 0, 0.125, 32.0, 12, P, 0, None) == _lib.MC_EUNSUPPORTED
