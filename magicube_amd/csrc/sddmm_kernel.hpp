@@ -101,7 +101,6 @@ __device__ __forceinline__ v4i expand_nibbles(v4i x, bool sext, u32 xor_mask) {
     return v4i{(int)e0, (int)o0, (int)e1, (int)o1};
 }
 
-constexpr int kSddmmStage = 1024;
 constexpr u32 kSddmmOob = 0x40000000u;
 
 template <int V, int BITS, bool SGN, int NKC, int W, int TPW, bool OUT_REF>
@@ -115,7 +114,6 @@ __global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
     constexpr u32 XHI = SGN ? 0u : 0x80808080u, XLO = 0x80808080u;
     constexpr int HOME = V < 4 ? V : 4;
     constexpr int KB = 64 * BITS / 8;          // bytes of one 64-k chunk
-    __shared__ __attribute__((aligned(16))) u32 s_off[kSddmmStage];
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -188,29 +186,28 @@ __global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
             }
         }
 
-        for (int c0 = start; c0 < wend; c0 += kSddmmStage) {
-            const int cn = min(wend - c0, kSddmmStage);
-            __syncthreads();
-            for (int i = threadIdx.x; i < ((cn + 15) & ~15); i += 64 * W) {
-                u32 o = kSddmmOob;
-                if (c0 + i < end) {
-                    const int idx = args.col_indices[c0 + i];
-                    if (idx >= 0) o = (u32)idx * row_bytes;
-                }
-                s_off[i] = o;
-            }
-            __syncthreads();
-            const int ctiles = (cn + 15) >> 4;
+        {
+            const int ctiles = (wend - start + 15) >> 4;
+            const __amdgpu_buffer_rsrc_t rc = make_rsrc(args.col_indices + start, (u32)(end - start) * 4u);
             // wave w handles tile groups w, w + W, ... of TPW tiles each
             for (int tg = wave * TPW; tg < ctiles; tg += W * TPW) {
                 constexpr int BD = W16 ? 2 : 1;   // rhs digits (16-bit: low, high)
                 v4i bop[TPW][NKC][BD];
                 int bsum[TPW];                    // sum of rhs' over this lane's k (16-bit)
+                // each wave loads the column indices of its own tiles (lane r:
+                // slot 16 tile + r) as rhs row byte offsets, out of range past
+                // the row end and for tiles past the row's last
+                u32 row_off[TPW];
 #pragma unroll
                 for (int t = 0; t < TPW; ++t) {
-                    const int tile = min(tg + t, ctiles - 1);
-                    const u32 add = (tg + t < ctiles) ? 0u : kSddmmOob;
-                    const u32 o = s_off[tile * 16 + r] + add;
+                    const int rel = (tg + t) * 16 + r;   // slot - start
+                    const bool in = tg + t < ctiles && start + rel < end;
+                    const int idx = (int)buf_load_b32(rc, in ? (u32)rel * 4u : 0x80000000u);
+                    row_off[t] = in && idx >= 0 ? (u32)idx * row_bytes : kSddmmOob;
+                }
+#pragma unroll
+                for (int t = 0; t < TPW; ++t) {
+                    const u32 o = row_off[t];
                     bsum[t] = 0;
 #pragma unroll
                     for (int kc = 0; kc < NKC; ++kc) {
@@ -274,7 +271,7 @@ __global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
                         }
                         res[i] = (int)vsum;
                     }
-                    const int j = c0 + (tg + t) * 16 + r;
+                    const int j = start + (tg + t) * 16 + r;
                     if (g < (V + 3) / 4 && tg + t < ctiles && j < wend) {
                         auto store_each = [&]() {
 #pragma unroll
