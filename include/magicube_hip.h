@@ -196,8 +196,8 @@ mc_status mc_sddmm_batched(int bits, mc_signedness sign, int layout, int batch, 
  *   Qq, Kq, Vq = trunc(clamp(x * scale))          (quantization_kernel.cu:46-74)
  *                or rint(clamp(x * scale)) with MC_ROUND_NEAREST; the clamp
  *                range is that of the output width, or [-8, 7] for int8
- *                output with MC_QUANT_RANGE4 (4-bit values, e.g. V of the
- *                reference's rhs_pre 4 layer, stored one per byte)
+ *                output with MC_QUANT_RANGE4 (4-bit values stored one per
+ *                byte; the fused launch also takes 4-bit V packed, v_bits 4)
  *   S = Qq Kq^T at the mask (int32, exact)
  *   s = float(S) * alpha     alpha = 1 / (scale_q * scale_k * sqrt(64))
  *   P, O by the requantization mode:
@@ -207,7 +207,7 @@ mc_status mc_sddmm_batched(int bits, mc_signedness sign, int layout, int batch, 
  *                     O = (float(P Vq) * o_scale) * (1 / sum)
  *     MC_P_REFERENCE_4BIT  as MC_P_REFERENCE with P clamped at 7: the
  *                     reference's 4-bit P (q_softmax_kernel.cu:84-118, lhs_pre 4),
- *                     held one value per byte
+ *                     written one value per byte where P is an output
  * The softmax statistics are accumulated over 64-slot chunks in a fixed
  * order, shared by every kernel below, so the one-op-per-launch path and the
  * fused batched path produce identical int32 S, int8 P and int32 P Vq. */
@@ -278,7 +278,7 @@ mc_status mc_attn_softmax_f16(int batch, int rows, int vec_length, const int *ro
 
 /* One attention matrix of the fused batched launch (device-resident
  * descriptor).  q, k: [L][64] int8 (or int4 packed [L][32] bytes); v: [L][64]
- * int8; o: [L][64] float.  o_int (int32 P Vq), p_out (P in the SpMM A layout)
+ * int8, or int4 packed [L][32] bytes with v_bits 4; o: [L][64] float.  o_int (int32 P Vq), p_out (P in the SpMM A layout)
  * and s_out (int32 S, [slot][V]) are optional debug outputs.  L <= 2^24 (the
  * kernel addresses rows with 32-bit byte offsets and marks masked slots with
  * offsets >= 2^31). */
@@ -308,12 +308,15 @@ typedef struct mc_attn_matrix {
  * i / 4, and consecutive workgroups go to the 8 XCDs in turn, so an order
  * that keeps each matrix on one XCD (see the Python package) keeps its K/V
  * rows in that XCD's L2.
+ * (qk_bits, v_bits) is (8, 8), (4, 8) or (4, 4): 4-bit V packed as the
+ * reference's rhs_pre 4 layer holds it (its bspmm_8b4b / bspmm_4b operand,
+ * deq_spmm.cpp:50-118); MC_EUNSUPPORTED otherwise.
  * debug != 0 also writes the descriptors' non-null o_int / p_out / s_out
  * (a slower kernel); with debug == 0 they are ignored.  No reference
  * counterpart: the reference launches each op separately with one mask
  * shared by the batch (spattention.py:189-210). */
 mc_status mc_attn_forward(const mc_attn_matrix *matrices, const int *items, int n_items, int qk_bits,
-                          int vec_length, float p_scale, int p_mode, int grid_waves, int debug,
+                          int v_bits, int vec_length, float p_scale, int p_mode, int grid_waves, int debug,
                           hipStream_t stream);
 
 /* ---- Host-side format preparation (no device work) ----------------------

@@ -40,6 +40,7 @@ P_REFERENCE_4BIT = 2        # P_REFERENCE with P clamped at 7 (4-bit P)
 ROUND_TRUNC = 0
 ROUND_NEAREST = 1
 QUANT_RANGE4 = 2            # or-ed into a rounding mode: 4-bit values stored as int8
+MAX_QUANT_JOBS = 65535      # jobs of one mc_attn_quantize_batched launch
 SIGNED = _lib.MC_SIGNED
 
 
@@ -170,6 +171,7 @@ class _Dev:
     alpha: float
     o_scale: float
     qk_bits: int
+    v_bits: int = 8
     o_int: torch.Tensor | None = None
     p_out: torch.Tensor | None = None
     s_out: torch.Tensor | None = None
@@ -193,15 +195,15 @@ class _Dev:
                                     *_rows(self.v16)))
 
 
-def working_set_bytes(L: int, qk_bits: int) -> int:
+def working_set_bytes(L: int, qk_bits: int, v_bits: int = 8) -> int:
     """Device bytes a matrix allocates when it joins a batch: quantized q, k,
     v and the fp32 output."""
     qk_row = HEAD_DIM * qk_bits // 8
-    return 2 * max(L * qk_row, 16) + max(L * HEAD_DIM, 16) + L * HEAD_DIM * 4
+    return 2 * max(L * qk_row, 16) + max(L * HEAD_DIM * v_bits // 8, 16) + L * HEAD_DIM * 4
 
 
 def make_resident(m: AttentionMatrix, scales: tuple, qk_bits: int, V: int, p_scale: float, device,
-                  debug: bool = False, out: torch.Tensor | None = None) -> _Dev:
+                  debug: bool = False, out: torch.Tensor | None = None, v_bits: int = 8) -> _Dev:
     """Uploads and aligns the mask when it is a host mask, and allocates the
     matrix's quantized operands and output, or writes into ``out``: a
     contiguous fp32 [L, 64] tensor, or an fp16 [L, 64] view with unit column
@@ -228,9 +230,9 @@ def make_resident(m: AttentionMatrix, scales: tuple, qk_bits: int, V: int, p_sca
              row_nnz=mask.row_nnz, q16=_as_input(m.q), k16=_as_input(m.k), v16=_as_input(m.v),
              qq=torch.empty(max(m.L * qk_row, 16), dtype=torch.uint8, device=device),
              kq=torch.empty(max(m.L * qk_row, 16), dtype=torch.uint8, device=device),
-             vq=torch.empty(max(m.L * HEAD_DIM, 16), dtype=torch.uint8, device=device),
+             vq=torch.empty(max(m.L * HEAD_DIM * v_bits // 8, 16), dtype=torch.uint8, device=device),
              o=out if out is not None else torch.empty((m.L, HEAD_DIM), dtype=torch.float32, device=device),
-             scales=(sq, sk, sv), alpha=alpha, o_scale=o_scale, qk_bits=qk_bits)
+             scales=(sq, sk, sv), alpha=alpha, o_scale=o_scale, qk_bits=qk_bits, v_bits=v_bits)
     if debug:
         an = mask.aligned_num
         d.o_int = torch.empty((m.L, HEAD_DIM), dtype=torch.int32, device=device)
@@ -331,21 +333,26 @@ class AttentionBatch:
     def __init__(self, mats: list[AttentionMatrix], qk_bits: int = 8, V: int = 8, p_mode: int = P_ROW_SCALED,
                  p_scale: float = 127.0, scales: list[tuple] | None = None, debug: bool = False,
                  rounding: int | None = None, device="cuda", resident: list[_Dev] | None = None,
-                 outputs: list[torch.Tensor] | None = None, v_range4: bool = False):
-        """v_range4: V is quantized to the 4-bit range [-8, 7] (kept one value
-        per byte), as the reference's rhs_pre 4 layer quantizes V."""
+                 outputs: list[torch.Tensor] | None = None, v_range4: bool = False, v_bits: int = 8):
+        """v_bits 4: V is quantized to 4 bits and held packed ([L][32] bytes),
+        as the reference's rhs_pre 4 layer holds it (Q/K must be 4-bit too).
+        v_range4: 8-bit V storage with values clamped to the 4-bit range
+        [-8, 7] (the same integers, one per byte)."""
         if qk_bits not in (8, 4):
             raise ValueError("qk_bits must be 8 or 4")
+        if v_bits not in (8, 4) or (v_bits == 4 and qk_bits != 4):
+            raise ValueError("v_bits must be 8, or 4 with 4-bit Q/K")
         if p_mode not in (P_REFERENCE, P_ROW_SCALED, P_REFERENCE_4BIT):
             raise ValueError("unknown p_mode")
         self.qk_bits, self.V, self.p_mode, self.p_scale = qk_bits, V, p_mode, float(p_scale)
+        self.v_bits = v_bits
         self.debug = bool(debug)
         # the reference truncates (quantization_kernel.cu:65-69); the row-scaled layer rounds
         self.rounding = rounding if rounding is not None else (ROUND_NEAREST if p_mode == P_ROW_SCALED else ROUND_TRUNC)
-        self.v_rounding = self.rounding | (QUANT_RANGE4 if v_range4 else 0)
+        self.v_rounding = self.rounding | (QUANT_RANGE4 if v_range4 and v_bits == 8 else 0)
         self.device = torch.device(device)
         if resident is not None:
-            if any(d.qk_bits != qk_bits for d in resident):
+            if any(d.qk_bits != qk_bits or d.v_bits != v_bits for d in resident):
                 raise ValueError("resident matrices were quantized for another width")
             self.mats = list(resident)
         else:
@@ -354,7 +361,7 @@ class AttentionBatch:
             outs = outputs if outputs is not None else [None] * len(mats)
             if len(outs) != len(mats):
                 raise ValueError("one output tensor per matrix")
-            self.mats = [make_resident(m, sc, qk_bits, V, self.p_scale, self.device, debug, o)
+            self.mats = [make_resident(m, sc, qk_bits, V, self.p_scale, self.device, debug, o, v_bits)
                          for m, sc, o in zip(mats, scales, outs)]
         self._assemble()
         self.graph = None
@@ -365,7 +372,7 @@ class AttentionBatch:
         quantization scales, width and output buffers are kept)."""
         if not devs:
             raise ValueError("empty batch")
-        return cls([], qk_bits=devs[0].qk_bits, resident=devs, **kw)
+        return cls([], qk_bits=devs[0].qk_bits, v_bits=devs[0].v_bits, resident=devs, **kw)
 
     def _assemble(self) -> None:
         """Work list (XCD-aware order, :func:`xcd_item_order`) and descriptors."""
@@ -375,8 +382,14 @@ class AttentionBatch:
         self.items = _pinned(order).to(dev, non_blocking=True)
         self.desc = _bytes_tensor(b"".join(d.desc_bytes() for d in self.mats), dev)
         jobs = [d.quant_job_bytes() for d in self.mats]
-        self.qk_jobs = _bytes_tensor(b"".join(j[0] for j in jobs), dev)
-        self.v_jobs = _bytes_tensor(b"".join(j[1] for j in jobs), dev)
+        qk_raw, v_raw = b"".join(j[0] for j in jobs), b"".join(j[1] for j in jobs)
+        self.qk_jobs = _bytes_tensor(qk_raw, dev)
+        self.v_jobs = _bytes_tensor(v_raw, dev)
+        # V at the width and rounding of Q and K joins their launch (the Q
+        # and K jobs, then the V jobs)
+        self.all_jobs = (_bytes_tensor(qk_raw + v_raw, dev)
+                         if self.v_bits == self.qk_bits and self.v_rounding == self.rounding
+                         and 3 * len(self.mats) <= MAX_QUANT_JOBS else None)
         self.max_count = max((d.L * HEAD_DIM for d in self.mats), default=0)
         # the descriptor copies run on the current stream; launches on any
         # other stream wait for them
@@ -394,15 +407,19 @@ class AttentionBatch:
         self._wait_ready(stream)
         lib, st = _lib.lib(), _stream(stream)
         n = len(self.mats)
+        if self.all_jobs is not None:
+            _lib.check(lib.mc_attn_quantize_batched(_dptr(self.all_jobs), 3 * n, self.qk_bits, self.rounding,
+                                                    self.max_count, st), "mc_attn_quantize_batched")
+            return
         _lib.check(lib.mc_attn_quantize_batched(_dptr(self.qk_jobs), 2 * n, self.qk_bits, self.rounding,
                                                 self.max_count, st), "mc_attn_quantize_batched")
-        _lib.check(lib.mc_attn_quantize_batched(_dptr(self.v_jobs), n, 8, self.v_rounding, self.max_count, st),
-                   "mc_attn_quantize_batched")
+        _lib.check(lib.mc_attn_quantize_batched(_dptr(self.v_jobs), n, self.v_bits, self.v_rounding,
+                                                self.max_count, st), "mc_attn_quantize_batched")
 
     def attend(self, stream=None, grid_waves: int = DEFAULT_GRID_WAVES) -> None:
         self._wait_ready(stream)
         _lib.check(_lib.lib().mc_attn_forward(_dptr(self.desc), _dptr(self.items), self.n_items, self.qk_bits,
-                                              self.V, self.p_scale, self.p_mode, grid_waves, int(self.debug),
+                                              self.v_bits, self.V, self.p_scale, self.p_mode, grid_waves, int(self.debug),
                                               _stream(stream)),
                    "mc_attn_forward")
 
@@ -441,9 +458,14 @@ class AttentionBatch:
             # fp32 result of the pipeline when the matrix's output is fp16
             w["o32"] = (d.o if d.o.dtype == torch.float32 else
                         torch.empty((d.L, HEAD_DIM), dtype=torch.float32, device=self.device))
+            # the pipeline's P V runs on int8 V: packed 4-bit V is quantized
+            # again into bytes clamped to [-8, 7] (the same integers)
+            if d.v_bits == 4:
+                w["vq8"] = torch.empty(max(d.L * HEAD_DIM, 16), dtype=torch.uint8, device=self.device)
+            vq8, v_rnd = (w["vq8"], self.rounding | QUANT_RANGE4) if d.v_bits == 4 else (d.vq, self.v_rounding)
             for name, (x, s, y, b, rnd) in {"jq": (d.q16, d.scales[0], d.qq, self.qk_bits, self.rounding),
                                             "jk": (d.k16, d.scales[1], d.kq, self.qk_bits, self.rounding),
-                                            "jv": (d.v16, d.scales[2], d.vq, 8, self.v_rounding)}.items():
+                                            "jv": (d.v16, d.scales[2], vq8, 8, v_rnd)}.items():
                 w[name] = (_struct_tensor([_QuantJobC(x.data_ptr(), x.numel(), s, y.data_ptr(), *_rows(x))],
                                           self.device), b, rnd)
         return w
@@ -466,8 +488,8 @@ class AttentionBatch:
                                                    _dptr(w["s"]), d.alpha, self.p_scale, self.p_mode,
                                                    _dptr(w["p"]), _dptr(w["inv_l"]), st), "softmax")
             _lib.check(lib.mc_spmm_run(8, 8, SIGNED, -1, d.m_vec, V, HEAD_DIM, d.L, _dptr(d.row_indices),
-                                       _dptr(d.row_offsets), _dptr(d.col_indices), _dptr(w["p"]), _dptr(d.vq),
-                                       _dptr(w["o_int"]), st), "spmm")
+                                       _dptr(d.row_offsets), _dptr(d.col_indices), _dptr(w["p"]),
+                                       _dptr(w.get("vq8", d.vq)), _dptr(w["o_int"]), st), "spmm")
             row_scale = _dptr(w["inv_l"]) if self.p_mode == P_ROW_SCALED else None
             _lib.check(lib.mc_attn_dequant(_dptr(w["o_int"]), d.L, HEAD_DIM, d.o_scale, row_scale, _dptr(w["o32"]),
                                            st), "dequant")
@@ -589,8 +611,13 @@ class BlockBatch:
         # record layout (numpy drops the padding of structured dtypes when it
         # promotes them)
         self.desc = _bytes_tensor(b"".join(b.desc.tobytes() for b in blocks), dev)
-        self.qk_jobs = _bytes_tensor(b"".join(b.qk_jobs.tobytes() for b in blocks), dev)
-        self.v_jobs = _bytes_tensor(b"".join(b.v_jobs.tobytes() for b in blocks), dev)
+        qk_raw = b"".join(b.qk_jobs.tobytes() for b in blocks)
+        v_raw = b"".join(b.v_jobs.tobytes() for b in blocks)
+        self.qk_jobs = _bytes_tensor(qk_raw, dev)
+        self.v_jobs = _bytes_tensor(v_raw, dev)
+        # int8 V joins the Q and K launch when they are int8 too
+        self.all_jobs = (_bytes_tensor(qk_raw + v_raw, dev)
+                         if self.qk_bits == 8 and 3 * self.n_mats <= MAX_QUANT_JOBS else None)
         self.max_count = max(b.max_count for b in blocks)
         # block j's XCD lists rotate by j so the first (largest) bins of the
         # requests spread over the XCDs; matrix indices shift to the batch's
@@ -616,11 +643,15 @@ class BlockBatch:
         if self._ready is not None and not torch.cuda.is_current_stream_capturing():
             (stream if stream is not None else torch.cuda.current_stream()).wait_event(self._ready)
         lib, st = _lib.lib(), _stream(stream)
-        _lib.check(lib.mc_attn_quantize_batched(_dptr(self.qk_jobs), 2 * self.n_mats, self.qk_bits, self.rounding,
-                                                self.max_count, st), "mc_attn_quantize_batched")
-        _lib.check(lib.mc_attn_quantize_batched(_dptr(self.v_jobs), self.n_mats, 8, self.rounding, self.max_count,
-                                                st), "mc_attn_quantize_batched")
-        _lib.check(lib.mc_attn_forward(_dptr(self.desc), _dptr(self.items), self.n_items, self.qk_bits, self.V,
+        if self.all_jobs is not None:
+            _lib.check(lib.mc_attn_quantize_batched(_dptr(self.all_jobs), 3 * self.n_mats, 8, self.rounding,
+                                                    self.max_count, st), "mc_attn_quantize_batched")
+        else:
+            _lib.check(lib.mc_attn_quantize_batched(_dptr(self.qk_jobs), 2 * self.n_mats, self.qk_bits,
+                                                    self.rounding, self.max_count, st), "mc_attn_quantize_batched")
+            _lib.check(lib.mc_attn_quantize_batched(_dptr(self.v_jobs), self.n_mats, 8, self.rounding,
+                                                    self.max_count, st), "mc_attn_quantize_batched")
+        _lib.check(lib.mc_attn_forward(_dptr(self.desc), _dptr(self.items), self.n_items, self.qk_bits, 8, self.V,
                                        self.p_scale, self.p_mode, grid_waves, 0, st), "mc_attn_forward")
 
 

@@ -339,13 +339,17 @@ __device__ __forceinline__ int const_load(const int *p) {
 // MFMA and keeps the online softmax statistics; pass 2 recomputes the exact
 // logits, quantizes P, moves it to the A layout through LDS, gathers V rows
 // (4x4 byte transposes) and accumulates O = P V with MFMA.
+// VB = 4: V rows are packed 4-bit ([L][32] bytes, lane 0 low); each lane
+// gathers the 16 bits of its four columns and widens them to the bytes
+// V + 8 (0..15), so the MFMA forms P (V + 8) and 8 * (row sum of P) comes off
+// the accumulators at the end (exact mod 2^32).
 // ---------------------------------------------------------------------------
 // DBG instantiations also write the optional int32 S, int8 P and int32 P V
 // outputs; the production ones carry no stores inside the chunk loops (a
 // conditional store there would make every load wait drain the queue).
 // Four waves per SIMD: the register budget is capped at 128 (a few
 // spills), 7 % faster than three waves with the uncapped allocation.
-template <int V, int QKB, int WPB, bool DBG>
+template <int V, int QKB, int VB, int WPB, bool DBG>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) void attention_kernel(const AttnMatrix *mats, const int2 *items, int n_items,
                                                              float p_scale, int p_mode) {
     constexpr int HOME = V < 4 ? V : 4;
@@ -395,7 +399,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
         const int nch_w = (aend - start + 63) >> 6;
         const u32 kv_bytes = (u32)M.L * QROW;
         const __amdgpu_buffer_rsrc_t rk = make_rsrc(M.k, kv_bytes);
-        const __amdgpu_buffer_rsrc_t rv = make_rsrc(M.v, (u32)M.L * kHeadDim);
+        const __amdgpu_buffer_rsrc_t rv = make_rsrc(M.v, (u32)M.L * (kHeadDim * VB / 8));
         const __amdgpu_buffer_rsrc_t rq = make_rsrc(M.q, kv_bytes);
 
         // SDDMM A operand: Q rows of this vector row (rows >= V read out of range -> 0)
@@ -433,12 +437,18 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
                 kop[t] = load_k16<QKB>(rk, row + (u32)(KCB * g));
             }
         };
-        // V rows of slots 16g + p (p = 0..15), dword c (columns 4c..4c+3)
+        // V rows of slots 16g + p (p = 0..15), columns 4c..4c+3: dword c of an
+        // int8 row, or the 16 bits at byte 2c of a packed 4-bit row
         auto load_vrows = [&](u32 my_off, u32 (&raw)[16]) {
 #pragma unroll
             for (int p = 0; p < 16; ++p) {
-                const u32 row = (u32)__shfl((int)my_off, 16 * g + p, 64);
-                raw[p] = buf_load_b32(rv, row + 4u * c);
+                u32 row = (u32)__shfl((int)my_off, 16 * g + p, 64);
+                if constexpr (VB == 4) {
+                    row = (row >> 1) | (row & kRowOob);   // 32-byte rows, marker kept
+                    raw[p] = buf_load_u16(rv, row + 2u * c);
+                } else {
+                    raw[p] = buf_load_b32(rv, row + 4u * c);
+                }
             }
         };
         // C-tile rows 4g + i of one 16-slot tile -> this lane's rows row0 + j
@@ -520,7 +530,8 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
         v4i oacc[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) oacc[b] = v4i{0, 0, 0, 0};
-        auto pass2_chunk = [&](int ch, const v4i (&kop)[4], const u32 (&raw)[16]) {
+        int psum = 0;   // VB = 4: sum of P row c over this lane's slots 16g..16g+15 of every chunk
+        auto pass2_chunk = [&](int ch, const v4i (&kop)[4], const u32 (&raw_in)[16]) {
             float s[4][HC];
             int si[4][4];
             logits(ch, kop, s, si);
@@ -530,6 +541,18 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
 #pragma unroll
                 for (int j = 0; j < HC; ++j) pq[t][j] = quantize_prob(s[t][j], mq[j], inv_l[j], p_scale, p_mode);
             const v4i pop = p_to_a_layout<HC>(tiles[wave], pq, lane, chome, row0);
+            u32 raw[16];
+#pragma unroll
+            for (int p = 0; p < 16; ++p) {
+                if constexpr (VB == 4) {
+                    // nibbles n0..n3 of V -> bytes n0 + 8 .. n3 + 8
+                    const u32 x = raw_in[p] ^ 0x8888u;
+                    raw[p] = perm((x >> 4) & 0x0F0Fu, x & 0x0F0Fu, 0x05010400u);
+                } else {
+                    raw[p] = raw_in[p];
+                }
+            }
+            if constexpr (VB == 4) psum += sum_bytes(pop);
             if (DBG && M.p_out) {
                 const int slot0 = start + 64 * ch + 16 * g;
                 if (c < V && slot0 < aend)
@@ -571,6 +594,20 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
         for (int i = 0; i < HOME; ++i) {
             const int src_group = V == 8 ? (g | (i & 2)) : ((4 * g + i) & 3);
             row_inv_l[i] = __shfl(inv_l[V == 8 ? (i & 1) : 0], 16 * src_group + c, 64);
+        }
+        if constexpr (VB == 4) {
+            // row c's sum of P over all slots (the four lane groups hold
+            // slots 16g..16g+15 of every chunk), then 8 * sum(P) of output
+            // row 4g + i off its accumulators
+            int s = psum;
+            s += lane_read(s, lane ^ 16);
+            s += lane_read(s, lane ^ 32);
+#pragma unroll
+            for (int i = 0; i < HOME; ++i) {
+                const u32 corr = 8u * (u32)lane_read(s, 4 * g + i);
+#pragma unroll
+                for (int b = 0; b < 4; ++b) oacc[b][i] = (int)((u32)oacc[b][i] - corr);
+            }
         }
         // rows 4g+i, columns 4c + b
         if (home) {
@@ -715,9 +752,10 @@ mc_status mc_attn_softmax_requant(int m_vec, int vec_length, const int *row_indi
     return mc::check_launch();
 }
 
-mc_status mc_attn_forward(const mc_attn_matrix *matrices, const int *items, int n_items, int qk_bits, int vec_length,
-                          float p_scale, int p_mode, int grid_waves, int debug, hipStream_t stream) {
+mc_status mc_attn_forward(const mc_attn_matrix *matrices, const int *items, int n_items, int qk_bits, int v_bits,
+                          int vec_length, float p_scale, int p_mode, int grid_waves, int debug, hipStream_t stream) {
     if (qk_bits != 8 && qk_bits != 4) return MC_EUNSUPPORTED;
+    if (v_bits != 8 && !(v_bits == 4 && qk_bits == 4)) return MC_EUNSUPPORTED;
     if (p_mode != MC_P_REFERENCE && p_mode != MC_P_ROW_SCALED && p_mode != MC_P_REFERENCE_4BIT) return MC_EINVAL_ARG;
     if (!mc::valid_v(vec_length)) return MC_EINVAL_V;
     if (n_items < 0 || grid_waves < 0) return MC_EINVAL_ARG;
@@ -729,28 +767,25 @@ mc_status mc_attn_forward(const mc_attn_matrix *matrices, const int *items, int 
     const dim3 grid((unsigned)((waves + WPB - 1) / WPB)), block(64 * WPB);
     const auto *mats = reinterpret_cast<const mc::AttnMatrix *>(matrices);
     const auto *its = reinterpret_cast<const int2 *>(items);
-#define MC_ATTN_LAUNCH(V_, B_)                                                                                       \
+#define MC_ATTN_LAUNCH(V_, B_, VB_)                                                                                  \
     do {                                                                                                             \
         if (debug)                                                                                                   \
-            hipLaunchKernelGGL((mc::attention_kernel<V_, B_, WPB, true>), grid, block, 0, stream, mats, its, n_items, \
-                               p_scale, p_mode);                                                                     \
+            hipLaunchKernelGGL((mc::attention_kernel<V_, B_, VB_, WPB, true>), grid, block, 0, stream, mats, its,    \
+                               n_items, p_scale, p_mode);                                                            \
         else                                                                                                         \
-            hipLaunchKernelGGL((mc::attention_kernel<V_, B_, WPB, false>), grid, block, 0, stream, mats, its,        \
+            hipLaunchKernelGGL((mc::attention_kernel<V_, B_, VB_, WPB, false>), grid, block, 0, stream, mats, its,   \
                                n_items, p_scale, p_mode);                                                            \
     } while (0)
-    if (qk_bits == 8) {
-        switch (vec_length) {
-        case 2: MC_ATTN_LAUNCH(2, 8); break;
-        case 4: MC_ATTN_LAUNCH(4, 8); break;
-        default: MC_ATTN_LAUNCH(8, 8); break;
-        }
-    } else {
-        switch (vec_length) {
-        case 2: MC_ATTN_LAUNCH(2, 4); break;
-        case 4: MC_ATTN_LAUNCH(4, 4); break;
-        default: MC_ATTN_LAUNCH(8, 4); break;
-        }
+#define MC_ATTN_LAUNCH_V(B_, VB_)                                                                                    \
+    switch (vec_length) {                                                                                            \
+    case 2: MC_ATTN_LAUNCH(2, B_, VB_); break;                                                                       \
+    case 4: MC_ATTN_LAUNCH(4, B_, VB_); break;                                                                       \
+    default: MC_ATTN_LAUNCH(8, B_, VB_); break;                                                                      \
     }
+    if (qk_bits == 8) { MC_ATTN_LAUNCH_V(8, 8) }
+    else if (v_bits == 8) { MC_ATTN_LAUNCH_V(4, 8) }
+    else { MC_ATTN_LAUNCH_V(4, 4) }
+#undef MC_ATTN_LAUNCH_V
 #undef MC_ATTN_LAUNCH
     return mc::check_launch();
 }

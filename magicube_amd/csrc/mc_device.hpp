@@ -66,6 +66,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, u3
 __device__ __forceinline__ u32 buf_load_b32(__amdgpu_buffer_rsrc_t r, u32 off) {
     return (u32)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
 }
+__device__ __forceinline__ u32 buf_load_u16(__amdgpu_buffer_rsrc_t r, u32 off) {
+    return (u32)__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+}
 __device__ __forceinline__ v2i buf_load_b64(__amdgpu_buffer_rsrc_t r, u32 off) {
     return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
 }

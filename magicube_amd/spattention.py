@@ -28,8 +28,8 @@ four pairs (spattention.py:194-215):
   exceeds 32, so the reference's 16-bit P holds exactly the int8 P values and
   P V is the same exact int32 product (its bspmm_16b8b splits P into byte
   planes);
-* 8, 4: int4 Q and K; V quantized to the 4-bit range [-8, 7], as the
-  reference quantizes it (spattention.py:191), held one value per byte;
+* 8, 4: int4 Q, K and V (spattention.py:191), V held packed as the
+  reference's bspmm_8b4b operand ([L][32] bytes, deq_spmm.cpp:50-84);
 * 4, 4: as 8, 4 with P clamped at 7 (the reference's 4-bit P,
   q_softmax_kernel.cu:84-118).
 
@@ -171,7 +171,7 @@ class _Plan:
         return AttentionBatch(mats, qk_bits=self.rhs_pre, V=self.masks[0].V,
                               p_mode=P_REFERENCE_4BIT if self.lhs_pre == 4 else P_REFERENCE, p_scale=SCALE_SFMX,
                               scales=[(SCALE_QKV, SCALE_QKV, SCALE_QKV)] * n_mat, rounding=ROUND_TRUNC,
-                              device=self.device, outputs=outs, v_range4=self.rhs_pre == 4, debug=debug)
+                              device=self.device, outputs=outs, v_bits=self.rhs_pre, debug=debug)
 
     def forward_with_weights(self) -> torch.Tensor:
         """Runs the layer through the kernel that also writes P and returns

@@ -87,11 +87,13 @@ def test_fused_equals_plain(qk_bits, p_mode, V):
 @pytest.mark.parametrize("V", [8, 4, 2])
 def test_production_kernel_equals_plain(qk_bits, p_mode, V):
     """The fused kernel without debug outputs (the one callers run) gives the
-    plain pipeline's fp32 O bit for bit (4-bit P runs with V in the 4-bit range)."""
+    plain pipeline's fp32 O bit for bit (4-bit P runs with V in the 4-bit
+    range: packed 4-bit V with 4-bit Q/K, one value per byte with int8 Q/K)."""
     mats, host = make_mats([128, 704, 1024], V=V, sparsity=0.8, seed=3 * V + qk_bits + p_mode)
     p_scale = 127.0 if p_mode == attention.P_ROW_SCALED else 32.0
+    v4 = p_mode == attention.P_REFERENCE_4BIT
     b = attention.AttentionBatch(mats, qk_bits=qk_bits, V=V, p_mode=p_mode, p_scale=p_scale,
-                                 v_range4=p_mode == attention.P_REFERENCE_4BIT)
+                                 v_range4=v4 and qk_bits == 8, v_bits=4 if v4 and qk_bits == 4 else 8)
     print(f"production fused vs plain V={V} qk_bits={qk_bits} p_mode={p_mode}", flush=True)
     b.forward_plain()
     torch.cuda.synchronize()
@@ -100,6 +102,38 @@ def test_production_kernel_equals_plain(qk_bits, p_mode, V):
     torch.cuda.synchronize()
     for d, po in zip(b.mats, plain):
         assert np.array_equal(po.view(np.uint32), d.o.cpu().numpy().view(np.uint32))
+
+
+@pytest.mark.parametrize("p_mode", [attention.P_ROW_SCALED, attention.P_REFERENCE, attention.P_REFERENCE_4BIT])
+@pytest.mark.parametrize("V", [8, 4, 2])
+def test_packed_4bit_v_equals_byte_v(V, p_mode):
+    """Packed 4-bit V ([L][32] bytes, gathered 16 bits per lane and widened
+    to V + 8 with 8 * sum(P) taken off the accumulators) gives the int32 S,
+    P, P V and O of the same integers held one per byte, bit for bit; the
+    packed operand is the nibble packing of the byte one; and the packed
+    batch's one-op-per-launch pipeline equals its fused launch."""
+    mats, host = make_mats([96, 640, 1000], V=V, sparsity=0.8, seed=60 + V + p_mode)
+    p_scale = 127.0 if p_mode == attention.P_ROW_SCALED else 32.0
+    kw = dict(qk_bits=4, V=V, p_mode=p_mode, p_scale=p_scale, debug=True)
+    byte_v = attention.AttentionBatch(mats, v_range4=True, **kw)
+    packed = attention.AttentionBatch(mats, v_bits=4, scales=[d.scales for d in byte_v.mats], **kw)
+    assert packed.all_jobs is not None and byte_v.all_jobs is None     # V joins the Q/K launch
+    print(f"packed 4-bit V vs byte V: V={V} p_mode={p_mode}", flush=True)
+    byte_v.forward()
+    packed.forward()
+    torch.cuda.synchronize()
+    for d8, d4, o8, o4 in zip(byte_v.mats, packed.mats, _debug_outputs(byte_v), _debug_outputs(packed)):
+        v8 = d8.vq.cpu().numpy()[: d8.L * 64].view(np.int8).astype(np.int32)
+        raw = d4.vq.cpu().numpy()[: d4.L * 32]
+        nib = np.stack([raw & 15, raw >> 4], axis=1).ravel().astype(np.int32)
+        assert np.array_equal(np.where(nib > 7, nib - 16, nib), v8), "packed V is not the nibble packing"
+        for name, a, b in zip(("S", "P", "P V", "O"), o8, o4):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), name
+    fused = [d.o.cpu().numpy().copy() for d in packed.mats]
+    packed.forward_plain()
+    torch.cuda.synchronize()
+    for d, fo in zip(packed.mats, fused):
+        assert np.array_equal(d.o.cpu().numpy().view(np.uint32), fo.view(np.uint32)), "plain vs fused"
 
 
 def _check_against_restatement(b, host, qk_bits, p_mode, p_scale, v_bits, V):
@@ -155,7 +189,8 @@ def test_against_numpy_restatement(qk_bits, p_mode, p_scale, v_bits):
     if p_mode != attention.P_ROW_SCALED:
         scales = [(36.0, 36.0, 36.0)] * len(mats)        # the reference layer's fixed scale (spattention.py:51-53)
     b = attention.AttentionBatch(mats, qk_bits=qk_bits, V=V, p_mode=p_mode, p_scale=p_scale, scales=scales,
-                                 v_range4=v_bits == 4, debug=True)
+                                 v_range4=v_bits == 4 and qk_bits == 8, v_bits=4 if v_bits == 4 and qk_bits == 4 else 8,
+                                 debug=True)
     print(f"restatement qk_bits={qk_bits} p_mode={p_mode}", flush=True)
     b.forward()
     torch.cuda.synchronize()
