@@ -176,6 +176,8 @@ def _default_cfg(lhs, rhs, V, n, m_vec):
     """The library's default tile rule, restated (spmm_api.hip default_config)."""
     tc = 16 * (32 // rhs)
     tiles = -(-n // tc)
+    if tiles >= 8 and m_vec * (-(-tiles // 8)) * 8 >= 4096:
+        return 14 if m_vec * (-(-tiles // 8)) * 8 >= 8192 else 13
     tpb = 4 if tiles >= 4 else (2 if tiles >= 2 else 1)
     if m_vec * (-(-tiles // tpb)) * tpb < 2048:
         return 20 if tpb == 4 else 22
@@ -183,23 +185,29 @@ def _default_cfg(lhs, rhs, V, n, m_vec):
 
 
 def _default_cases():
-    """Seeded shapes, eight per default tile (15, 16, 17, 20, 22): ragged N
-    (any multiple of 32/rhs_bits), row counts on both sides of the 2048-wave
-    split, every pair and V."""
+    """Seeded shapes, eight per default tile (13, 14, 15, 16, 17, 20, 22):
+    ragged N (any multiple of 32/rhs_bits), row counts on both sides of the
+    2048-, 4096- and 8192-wave splits, every pair and V."""
     rng = np.random.default_rng(2024)
     cases = []
-    for i in range(40):
-        target = (15, 16, 17, 20, 22)[i % 5]
-        pair = PAIRS[(i // 5) % len(PAIRS)]
+    targets = (13, 14, 15, 16, 17, 20, 22)
+    for i in range(8 * len(targets)):
+        target = targets[i % len(targets)]
+        pair = PAIRS[(i // len(targets)) % len(PAIRS)]
         V = (2, 4, 8)[i % 3]
         cpd = 32 // pair[1]
         r = int(rng.integers(1, 17) if target == 15 else
-                rng.integers(17, 49) if target in (16, 22) else rng.integers(49, 97))
+                rng.integers(17, 49) if target in (16, 22) else
+                rng.integers(113, 161) if target in (13, 14) else rng.integers(49, 97))
         n = cpd * r
         tiles = -(-r // 16)
-        tpb = 4 if tiles >= 4 else (2 if tiles >= 2 else 1)
+        tpb = 8 if target in (13, 14) else 4 if tiles >= 4 else (2 if tiles >= 2 else 1)
         per_row = -(-tiles // tpb) * tpb
-        if target in (20, 22):
+        if target == 13:
+            m_vec = int(rng.integers(-(-4096 // per_row), 8192 // per_row))
+        elif target == 14:
+            m_vec = -(-8192 // per_row) + int(rng.integers(0, 100))
+        elif target in (20, 22):
             m_vec = int(rng.integers(1, 2048 // per_row))
         else:
             m_vec = -(-2048 // per_row) + int(rng.integers(0, 200))
@@ -211,7 +219,7 @@ def _default_cases():
 
 def test_default_rule_cases_cover_every_default_tile():
     got = {_default_cfg(*c[0], c[1], c[4], c[2]) for c in _default_cases()}
-    assert got == {15, 16, 17, 20, 22}, got
+    assert got == {13, 14, 15, 16, 17, 20, 22}, got
 
 
 @pytest.mark.parametrize("case", _default_cases(), ids=lambda c: f"L{c[0][0]}R{c[0][1]}-V{c[1]}-m{c[2]}-K{c[3]}-N{c[4]}-{'s' if c[6] else 'u'}")
