@@ -637,6 +637,10 @@ def attention_bench(steps, warmup, seed=101, samples=8, heads=16, V=8, sparsity=
     nnz = sum(m.nnz_vec for m in masks)
     ops = 2.0 * 2.0 * nnz * V * attention.HEAD_DIM
     alg = sum(3 * m.L * 64 * 2 + m.L * 64 * 4 + 4 * m.mask.nnz_vec + 4 * (m.mask.m_vec + 1) for m in mats)
+    # the one-op-per-launch pipeline also writes and reads back the int8
+    # Q/K/V, the int32 S, the int8 P and the int32 P V
+    unfused = alg + sum(2 * 3 * m.L * 64 + 2 * 4 * V * m.mask.nnz_vec + 2 * V * m.mask.nnz_vec + 2 * 4 * m.L * 64
+                        for m in mats)
 
     def fused():
         batch.forward(stream)
@@ -676,6 +680,8 @@ def attention_bench(steps, warmup, seed=101, samples=8, heads=16, V=8, sparsity=
         "speedup_vs_plain_eager": plain_eager_ms / fused_ms, "speedup_vs_plain_graph": plain_graph_ms / fused_ms,
         "fused_tops": ops / (fused_ms * 1e-3) / 1e12, "algorithmic_bytes": alg,
         "fused_hbm_frac": alg / (fused_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "unfused_bytes": unfused, "unfused_roofline_ms": unfused / HBM_PEAK_GBS / 1e6,
+        "plain_graph_hbm_frac": unfused / (plain_graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "fused_equals_plain": identical,
         "error_vs_fp64": {"max_abs": max(e[0] for e in errs), "rel_fro_max": max(e[1] for e in errs),
                           "rel_fro_mean": statistics.mean(e[1] for e in errs)},
