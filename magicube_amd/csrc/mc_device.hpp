@@ -50,6 +50,25 @@ __device__ __forceinline__ u32 sext4_bytes(u32 t) {
     return t | ((m << 5) - (m << 1));   // 30*m == 0xF0 per flagged byte, no carries
 }
 
+// 4 x 4 transpose of ints within each quad of lanes: on return lane q
+// (q = lane & 3) holds in x[k] what lane k of its quad held in x[q].  Two
+// butterfly stages (lane bit 0 with register bit 0, then bit 1 with bit 1),
+// each one DPP quad_perm swap per register pair.
+__device__ __forceinline__ void quad_transpose4(int (&x)[4], int lane) {
+    // written out with selects: a data-dependent choice of array element
+    // would move the array out of registers
+    const bool h0 = (lane & 1) != 0, h1 = (lane & 2) != 0;
+    const int g0 = __builtin_amdgcn_update_dpp(0, h0 ? x[0] : x[1], 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    const int g1 = __builtin_amdgcn_update_dpp(0, h0 ? x[2] : x[3], 0xB1, 0xF, 0xF, false);
+    const int y0 = h0 ? g0 : x[0], y1 = h0 ? x[1] : g0, y2 = h0 ? g1 : x[2], y3 = h0 ? x[3] : g1;
+    const int k0 = __builtin_amdgcn_update_dpp(0, h1 ? y0 : y2, 0x4E, 0xF, 0xF, false);      // quad_perm [2,3,0,1]
+    const int k1 = __builtin_amdgcn_update_dpp(0, h1 ? y1 : y3, 0x4E, 0xF, 0xF, false);
+    x[0] = h1 ? k0 : y0;
+    x[1] = h1 ? k1 : y1;
+    x[2] = h1 ? y2 : k0;
+    x[3] = h1 ? y3 : k1;
+}
+
 __device__ __forceinline__ int sum_bytes(v4i a) {
     int s = __builtin_amdgcn_sdot4(a[0], 0x01010101, 0, false);
     s = __builtin_amdgcn_sdot4(a[1], 0x01010101, s, false);

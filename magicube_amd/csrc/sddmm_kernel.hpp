@@ -272,6 +272,17 @@ __global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
                         res[i] = (int)vsum;
                     }
                     const int j = start + (tg + t) * 16 + r;
+                    // reference layout with four rows per lane: a 4 x 4 transpose
+                    // within each quad of lanes gives lane r row 4g + (r & 3) of
+                    // slots j4..j4+3 (j4 = j & ~3, start being a multiple of 8),
+                    // one 16-byte run of [(j/8)][v][j%8].  A quad's four lanes are
+                    // all in or all out of the store below (its slots share j4).
+                    int tq[4] = {0, 0, 0, 0};
+                    if constexpr (OUT_REF && HOME == 4) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) tq[i] = j < end ? res[i] : 0;
+                        quad_transpose4(tq, lane);
+                    }
                     if (g < (V + 3) / 4 && tg + t < ctiles && j < wend) {
                         auto store_each = [&]() {
 #pragma unroll
@@ -284,7 +295,22 @@ __global__ __launch_bounds__(64 * W) void sddmm_kernel(SddmmArgs args) {
                                 out[o] = ps == 0 ? val : (int)((u32)out[o] + (u32)val);
                             }
                         };
-                        if constexpr (!OUT_REF && HOME == 4) {
+                        if constexpr (OUT_REF && HOME == 4) {
+                            if ((reinterpret_cast<uintptr_t>(out) & 15u) == 0) {
+                                const int j4 = j & ~3;
+                                const int v = 4 * g + (r & 3);
+                                v4i val = v4i{tq[0], tq[1], tq[2], tq[3]};
+                                v4i *o = reinterpret_cast<v4i *>(out + (size_t)(j4 >> 3) * 8 * V + 8 * v + (j4 & 7));
+                                if (ps != 0) {
+                                    const v4i prev = *o;
+#pragma unroll
+                                    for (int i = 0; i < 4; ++i) val[i] = (int)((u32)prev[i] + (u32)val[i]);
+                                }
+                                *o = val;
+                            } else {
+                                store_each();
+                            }
+                        } else if constexpr (!OUT_REF && HOME == 4) {
                             if ((reinterpret_cast<uintptr_t>(out) & 15u) == 0) {
                                 // [slot][V]: the lane's rows 4g..4g+3 are one 16-byte run
                                 v4i val = j < end ? v4i{res[0], res[1], res[2], res[3]} : v4i{0, 0, 0, 0};
