@@ -7,8 +7,11 @@
 //                  waves per (vector row, 256-column half), each lane one
 //                  dword of the 16 B rows of its lane group's slots per
 //                  64-slot k-block, gather offsets staged in LDS per 1024
-//                  slots (spmm_kernel.hpp, default tile of this shape)
-//   gather_2deep   the same with two k-blocks of loads in flight
+//                  slots (spmm_kernel.hpp, tile config 17)
+//   gather_8w      one workgroup of eight waves per vector row (all 512
+//                  columns), otherwise the same (tile config 13, the default
+//                  tile of this shape)
+//   gather_2deep   the four-wave gather with two k-blocks of loads in flight
 //   gather_x4      one workgroup per vector row, 16 bytes per lane
 //   fill_strided   a B-stationary kernel's fill: 256 workgroups, each copying
 //                  a 32-column slice of all K rows of row-major B (32 of
@@ -183,6 +186,7 @@ int main(int argc, char **argv) {
            "\"b_row_segments_64B\": %zu}\n", nnz, nnz * (kN / 64));
     auto line = [](const char *name, double us) { printf("{\"probe\": \"%s\", \"host_us_per_launch\": %.3f}\n", name, us); };
     line("gather", mean_us([&] { hipLaunchKernelGGL((gather<1, 1, 4, 2>), dim3(kMVec * 2), dim3(256), 0, 0, dp, dc, dB, sink); }, reps));
+    line("gather_8w", mean_us([&] { hipLaunchKernelGGL((gather<1, 1, 8, 1>), dim3(kMVec), dim3(512), 0, 0, dp, dc, dB, sink); }, reps));
     line("gather_2deep", mean_us([&] { hipLaunchKernelGGL((gather<1, 2, 4, 2>), dim3(kMVec * 2), dim3(256), 0, 0, dp, dc, dB, sink); }, reps));
     line("gather_x4", mean_us([&] { hipLaunchKernelGGL((gather<4, 1, 2, 1>), dim3(kMVec), dim3(128), 0, 0, dp, dc, dB, sink); }, reps));
     line("fill_strided", mean_us([&] { hipLaunchKernelGGL(fill<false>, dim3(256), dim3(1024), 131072, 0, dB, sink); }, reps));
