@@ -136,6 +136,23 @@ def test_packed_4bit_v_equals_byte_v(V, p_mode):
         assert np.array_equal(d.o.cpu().numpy().view(np.uint32), fo.view(np.uint32)), "plain vs fused"
 
 
+def test_packed_4bit_v_long_sequence():
+    """The packed 4-bit V path at the serving trace's longest matrix (L = 8192,
+    80 % sparse, about 1640 keys per row, V = 8) in the reference's 4-bit
+    modes: S, P, P V and O equal those of byte-held V bit for bit."""
+    mats, host = make_mats([8192], V=8, sparsity=0.8, seed=71)
+    for p_mode in (attention.P_REFERENCE, attention.P_REFERENCE_4BIT):
+        kw = dict(qk_bits=4, V=8, p_mode=p_mode, p_scale=32.0, scales=[(36.0, 36.0, 36.0)], debug=True)
+        byte_v = attention.AttentionBatch(mats, v_range4=True, **kw)
+        packed = attention.AttentionBatch(mats, v_bits=4, **kw)
+        print(f"packed 4-bit V, L=8192, p_mode={p_mode}", flush=True)
+        byte_v.forward()
+        packed.forward()
+        torch.cuda.synchronize()
+        for name, a, b in zip(("S", "P", "P V", "O"), _debug_outputs(byte_v)[0], _debug_outputs(packed)[0]):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), name
+
+
 def _check_against_restatement(b, host, qk_bits, p_mode, p_scale, v_bits, V):
     """Every real slot of every vector row: int32 S exact, P within one step
     of the FP32 restatement (bitwise equal in most slots), P V from the
