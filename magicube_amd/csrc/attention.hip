@@ -349,7 +349,7 @@ __device__ __forceinline__ int const_load(const int *p) {
 // conditional store there would make every load wait drain the queue).
 // Four waves per SIMD: the register budget is capped at 128 (a few
 // spills), 7 % faster than three waves with the uncapped allocation.
-template <int V, int QKB, int VB, int WPB, bool DBG>
+template <int V, int QKB, int VB, int PM, int WPB, bool DBG>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) void attention_kernel(const AttnMatrix *mats, const int2 *items, int n_items,
                                                              float p_scale, int p_mode) {
     constexpr int HOME = V < 4 ? V : 4;
@@ -539,7 +539,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
 #pragma unroll
             for (int t = 0; t < 4; ++t)
 #pragma unroll
-                for (int j = 0; j < HC; ++j) pq[t][j] = quantize_prob(s[t][j], mq[j], inv_l[j], p_scale, p_mode);
+                for (int j = 0; j < HC; ++j) pq[t][j] = quantize_prob<PM>(s[t][j], mq[j], inv_l[j], p_scale, p_mode);
             const v4i pop = p_to_a_layout<HC>(tiles[wave], pq, lane, chome, row0);
             u32 raw[16];
 #pragma unroll
@@ -617,7 +617,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
                 if (DBG && M.o_int)
                     *reinterpret_cast<v4i *>(M.o_int + orow) = v4i{oacc[0][i], oacc[1][i], oacc[2][i], oacc[3][i]};
                 // same float operations as dequant_kernel with/without a row scale
-                const float rs = p_mode == kPModeRowScaled ? row_inv_l[i] : 1.f;
+                const float rs = PM == kPModeRowScaled ? row_inv_l[i] : 1.f;
                 float4 of;
                 of.x = ((float)oacc[0][i] * M.o_scale) * rs;
                 of.y = ((float)oacc[1][i] * M.o_scale) * rs;
@@ -767,24 +767,31 @@ mc_status mc_attn_forward(const mc_attn_matrix *matrices, const int *items, int 
     const dim3 grid((unsigned)((waves + WPB - 1) / WPB)), block(64 * WPB);
     const auto *mats = reinterpret_cast<const mc::AttnMatrix *>(matrices);
     const auto *its = reinterpret_cast<const int2 *>(items);
-#define MC_ATTN_LAUNCH(V_, B_, VB_)                                                                                  \
+#define MC_ATTN_LAUNCH(V_, B_, VB_, PM_)                                                                             \
     do {                                                                                                             \
         if (debug)                                                                                                   \
-            hipLaunchKernelGGL((mc::attention_kernel<V_, B_, VB_, WPB, true>), grid, block, 0, stream, mats, its,    \
-                               n_items, p_scale, p_mode);                                                            \
+            hipLaunchKernelGGL((mc::attention_kernel<V_, B_, VB_, PM_, WPB, true>), grid, block, 0, stream, mats,    \
+                               its, n_items, p_scale, p_mode);                                                       \
         else                                                                                                         \
-            hipLaunchKernelGGL((mc::attention_kernel<V_, B_, VB_, WPB, false>), grid, block, 0, stream, mats, its,   \
-                               n_items, p_scale, p_mode);                                                            \
+            hipLaunchKernelGGL((mc::attention_kernel<V_, B_, VB_, PM_, WPB, false>), grid, block, 0, stream, mats,   \
+                               its, n_items, p_scale, p_mode);                                                       \
     } while (0)
+#define MC_ATTN_LAUNCH_P(V_, B_, VB_)                                                                                \
+    switch (p_mode) {                                                                                                \
+    case MC_P_REFERENCE: MC_ATTN_LAUNCH(V_, B_, VB_, 0); break;                                                      \
+    case MC_P_ROW_SCALED: MC_ATTN_LAUNCH(V_, B_, VB_, 1); break;                                                     \
+    default: MC_ATTN_LAUNCH(V_, B_, VB_, 2); break;                                                                  \
+    }
 #define MC_ATTN_LAUNCH_V(B_, VB_)                                                                                    \
     switch (vec_length) {                                                                                            \
-    case 2: MC_ATTN_LAUNCH(2, B_, VB_); break;                                                                       \
-    case 4: MC_ATTN_LAUNCH(4, B_, VB_); break;                                                                       \
-    default: MC_ATTN_LAUNCH(8, B_, VB_); break;                                                                      \
+    case 2: MC_ATTN_LAUNCH_P(2, B_, VB_); break;                                                                     \
+    case 4: MC_ATTN_LAUNCH_P(4, B_, VB_); break;                                                                     \
+    default: MC_ATTN_LAUNCH_P(8, B_, VB_); break;                                                                    \
     }
     if (qk_bits == 8) { MC_ATTN_LAUNCH_V(8, 8) }
     else if (v_bits == 8) { MC_ATTN_LAUNCH_V(4, 8) }
     else { MC_ATTN_LAUNCH_V(4, 4) }
+#undef MC_ATTN_LAUNCH_P
 #undef MC_ATTN_LAUNCH_V
 #undef MC_ATTN_LAUNCH
     return mc::check_launch();

@@ -96,10 +96,14 @@ __device__ __forceinline__ float finite_row_max(float m) { return m == -INFINITY
 
 // Quantized probability of one logit given the final row statistics
 // (m from finite_row_max); a masked slot (s = -inf) gives exp(-inf) = 0.
+// PM fixed at compile time (the fused kernel's chunk loops), or -1 for a
+// run-time p_mode; both evaluate the same expressions.
+template <int PM = -1>
 __device__ __forceinline__ int quantize_prob(float s, float m, float inv_l, float p_scale, int p_mode) {
+    const int mode = PM >= 0 ? PM : p_mode;
     const float e = __expf(s - m);
-    if (p_mode != kPModeRowScaled)   // truncation (p >= 0)
-        return (int)fminf(e * inv_l * p_scale, p_mode == kPModeReference4 ? 7.f : 127.f);
+    if (mode != kPModeRowScaled)   // truncation (p >= 0)
+        return (int)fminf(e * inv_l * p_scale, mode == kPModeReference4 ? 7.f : 127.f);
     return (int)rintf(fminf(e * p_scale, 127.f));
 }
 
