@@ -319,6 +319,17 @@ mc_status mc_attn_forward(const mc_attn_matrix *matrices, const int *items, int 
                           int v_bits, int vec_length, float p_scale, int p_mode, int grid_waves, int debug,
                           hipStream_t stream);
 
+/* LayerNorm over rows of cols fp16 values (cols 256, 512 or 1024), fused
+ * with the residual add before it when residual is non-null:
+ *   s = half(x + residual),  out = half((s - mean) * rsqrt(var + eps) * weight + bias)
+ * mean and the biased variance of the row in fp32; weight / bias (fp16, cols
+ * values) may be null (1 and 0).  x, residual and out 8-byte aligned; out may
+ * alias x or residual.  No counterpart in the reference's library: its
+ * encoder uses torch's LayerNorm after a separate add
+ * (end2end_eval/sparse_transformer_magicube/end_to_end.py:77-89). */
+mc_status mc_layernorm_f16(long long rows, int cols, const void *x, const void *residual, const void *weight,
+                           const void *bias, float eps, void *out, hipStream_t stream);
+
 /* ---- Host-side format preparation (no device work) ----------------------
  * Builds the reference layouts from a plain vector-CSR (row_offsets has
  * m_vec+1 entries, col_indices nnz_vec entries, values nnz_vec packed

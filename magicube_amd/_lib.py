@@ -53,6 +53,7 @@ _SIGS["mc_attn_softmax_requant"] = (_I, [_I, _I, _P, _P, _P, _F, _F, _I, _P, _P,
 _SIGS["mc_attn_dequant"] = (_I, [_P, _I, _I, _F, _P, _P, _P])
 _SIGS["mc_attn_forward"] = (_I, [_P, _P, _I, _I, _I, _I, _F, _I, _I, _I, _P])
 _SIGS["mc_attn_dequant_f16"] = (_I, [_P, _LL, _F, _P, _P])
+_SIGS["mc_layernorm_f16"] = (_I, [_LL, _I, _P, _P, _P, _P, _F, _P, _P])
 _SIGS["mc_attn_softmax_f16"] = (_I, [_I, _I, _I, _P, _P, _P, _LL, _F, _F, _I, _P, _LL, _P])
 _SIGS["mc_spmm_batched"] = (_I, [_I, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _LL, _P, _LL, _P, _LL, _P])
 _SIGS["mc_sddmm_batched"] = (_I, [_I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _LL, _P, _LL, _P, _LL, _I, _P])

@@ -40,7 +40,7 @@ LIB_SOURCES = [
     "spmm_l12r4.hip",
     "spmm_l16r4.hip",
 ]
-OPTIONAL_SOURCES = ["sddmm.hip", "attention.hip"]
+OPTIONAL_SOURCES = ["sddmm.hip", "attention.hip", "layernorm.hip"]
 # Vendor-library comparators (rocSPARSE, hipBLASLt): a library of their own so
 # the product library carries no dependency on them.
 VENDOR_LIB = os.path.join(HERE, "libmagicube_vendor.so")

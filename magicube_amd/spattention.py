@@ -28,8 +28,11 @@ four pairs (spattention.py:194-215):
   exceeds 32, so the reference's 16-bit P holds exactly the int8 P values and
   P V is the same exact int32 product (its bspmm_16b8b splits P into byte
   planes);
-* 8, 4: int4 Q, K and V (spattention.py:191), V held packed as the
-  reference's bspmm_8b4b operand ([L][32] bytes, deq_spmm.cpp:50-84);
+* 8, 4: int4 Q, K and V (spattention.py:191); V held one 4-bit value per
+  byte: the fused kernel also takes it packed as the reference's bspmm_8b4b
+  operand ([L][32] bytes, deq_spmm.cpp:50-84, ``AttentionBatch(v_bits=4)``),
+  bit for bit the same, but its 16-bit gathers touch the same L1 lines and
+  add the widening, 27 % slower on a 128-matrix batch;
 * 4, 4: as 8, 4 with P clamped at 7 (the reference's 4-bit P,
   q_softmax_kernel.cu:84-118).
 
@@ -59,7 +62,7 @@ import torch
 import torch.nn.functional as F
 from torch import nn
 
-from . import formats
+from . import _lib, formats
 from .attention import (HEAD_DIM, P_REFERENCE, P_REFERENCE_4BIT, ROUND_TRUNC, AttentionBatch, AttentionMatrix,
                         ResidentMask)
 
@@ -171,7 +174,7 @@ class _Plan:
         return AttentionBatch(mats, qk_bits=self.rhs_pre, V=self.masks[0].V,
                               p_mode=P_REFERENCE_4BIT if self.lhs_pre == 4 else P_REFERENCE, p_scale=SCALE_SFMX,
                               scales=[(SCALE_QKV, SCALE_QKV, SCALE_QKV)] * n_mat, rounding=ROUND_TRUNC,
-                              device=self.device, outputs=outs, v_bits=self.rhs_pre, debug=debug)
+                              device=self.device, outputs=outs, v_range4=self.rhs_pre == 4, debug=debug)
 
     def forward_with_weights(self) -> torch.Tensor:
         """Runs the layer through the kernel that also writes P and returns
@@ -296,6 +299,34 @@ class SpMultiheadAttention(nn.MultiheadAttention):
         return F.linear(plan.o, self.out_proj.weight, self.out_proj.bias), weights
 
 
+class FusedLayerNorm(nn.LayerNorm):
+    """nn.LayerNorm over the last dimension whose fp16 inference forward is
+    one HIP launch (``mc_layernorm_f16``), with the residual add before it
+    fused when ``residual`` is given: LN(half(x + residual)).  Other inputs
+    (other dtypes or widths, autograd) take torch's own add and LayerNorm,
+    which the reference's encoder uses (end_to_end.py:77-89)."""
+
+    def forward(self, x, residual=None):
+        w, b = self.weight, self.bias
+        fast = (x.is_cuda and x.dtype == torch.float16 and x.is_contiguous() and len(self.normalized_shape) == 1
+                and self.normalized_shape[0] in (256, 512, 1024) and x.shape[-1] == self.normalized_shape[0]
+                and (w is None or (w.dtype == torch.float16 and w.is_cuda))
+                and (b is None or (b.dtype == torch.float16 and b.is_cuda))
+                and (residual is None or (residual.shape == x.shape and residual.dtype == x.dtype
+                                          and residual.is_cuda and residual.is_contiguous()))
+                and not (torch.is_grad_enabled() and (x.requires_grad or (w is not None and w.requires_grad))))
+        if not fast:
+            return super().forward(x if residual is None else x + residual)
+        import ctypes
+        out = torch.empty_like(x)
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+        _lib.check(_lib.lib().mc_layernorm_f16(x.numel() // x.shape[-1], x.shape[-1], ptr(x), ptr(residual), ptr(w),
+                                                ptr(b), float(self.eps), ptr(out),
+                                                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                   "mc_layernorm_f16")
+        return out
+
+
 class PositionalEncoding(nn.Module):
     """Sinusoidal positions added to [L, batch, embed] (end_to_end.py:115-130)."""
 
@@ -325,8 +356,8 @@ class SparseTransformerBlock(nn.Module):
         (self.column_indices, self.column_indices_shuffle, self.row_offsets, self.row_indices,
          self.aligned_num_item) = static_random_mask_aligned(seq_len // vec_length, seq_len, sparsity, mma_k_dim,
                                                              seed=seed, device=device)
-        self.layer_norm1 = nn.LayerNorm(normalized_shape=embed_dim)
-        self.layer_norm2 = nn.LayerNorm(normalized_shape=embed_dim)
+        self.layer_norm1 = FusedLayerNorm(normalized_shape=embed_dim)
+        self.layer_norm2 = FusedLayerNorm(normalized_shape=embed_dim)
         self.linear1 = nn.Linear(embed_dim, mlp_dim)
         self.linear2 = nn.Linear(mlp_dim, embed_dim)
 
@@ -335,8 +366,7 @@ class SparseTransformerBlock(nn.Module):
         out = self.self_attention(out, out, out, need_weights=False, row_indices=self.row_indices,
                                   row_offsets=self.row_offsets, column_indices=self.column_indices,
                                   vec_length=self.vec_length, lhs_pre=self.lhs_pre, rhs_pre=self.rhs_pre)
-        out = out[0] + x
-        out = self.layer_norm2(out)
+        out = self.layer_norm2(out[0], residual=x)      # LN(out + x), the add fused
         out = self.linear1(out)
         out = self.linear2(out)
         return out + x
@@ -349,16 +379,15 @@ class TransformerBlock(nn.Module):
     def __init__(self, embed_dim, mlp_dim, num_heads):
         super().__init__()
         self.self_attention = nn.MultiheadAttention(embed_dim=embed_dim, num_heads=num_heads, dropout=0)
-        self.layer_norm1 = nn.LayerNorm(normalized_shape=embed_dim)
-        self.layer_norm2 = nn.LayerNorm(normalized_shape=embed_dim)
+        self.layer_norm1 = FusedLayerNorm(normalized_shape=embed_dim)
+        self.layer_norm2 = FusedLayerNorm(normalized_shape=embed_dim)
         self.linear1 = nn.Linear(embed_dim, mlp_dim)
         self.linear2 = nn.Linear(mlp_dim, embed_dim)
 
     def forward(self, x):
         out = self.layer_norm1(x)
         out = self.self_attention(out, out, out, need_weights=False)
-        out = out[0] + x
-        out = self.layer_norm2(out)
+        out = self.layer_norm2(out[0], residual=x)
         out = self.linear1(out)
         out = self.linear2(out)
         return out + x
@@ -371,7 +400,7 @@ class _Encoder(nn.Module):
         self.pos_encoder = PositionalEncoding(embed_dim)
         self.blocks = nn.ModuleList(blocks)
         self.embed_dim = embed_dim
-        self.layer_norm = nn.LayerNorm(normalized_shape=embed_dim)
+        self.layer_norm = FusedLayerNorm(normalized_shape=embed_dim)
         self.linear = nn.Linear(embed_dim, num_class)
 
     def forward(self, x):

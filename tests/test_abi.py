@@ -161,6 +161,16 @@ def test_batched_and_reference_op_validation(mclib):
     assert mclib.mc_attn_dequant_f16(P, 0, 1.0, P, None) == _lib.MC_OK
 
 
+def test_layernorm_validation(mclib):
+    P = ctypes.c_void_p(16)
+    assert mclib.mc_layernorm_f16(4, 384, P, None, None, None, 1e-5, P, None) == _lib.MC_EINVAL_SHAPE
+    assert mclib.mc_layernorm_f16(-1, 256, P, None, None, None, 1e-5, P, None) == _lib.MC_EINVAL_SHAPE
+    assert mclib.mc_layernorm_f16(4, 256, None, None, None, None, 1e-5, P, None) == _lib.MC_EINVAL_ARG
+    assert mclib.mc_layernorm_f16(4, 256, ctypes.c_void_p(18), None, None, None, 1e-5, P, None) == _lib.MC_EINVAL_ARG
+    assert mclib.mc_layernorm_f16(4, 256, P, ctypes.c_void_p(20), None, None, 1e-5, P, None) == _lib.MC_EINVAL_ARG
+    assert mclib.mc_layernorm_f16(0, 256, None, None, None, None, 1e-5, None, None) == _lib.MC_OK
+
+
 def test_sptrans_rejects_host_tensors():
     """The reference-signature ops take device tensors only (no CPU path)."""
     from magicube_amd import sptrans
