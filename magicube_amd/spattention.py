@@ -317,13 +317,14 @@ class FusedLayerNorm(nn.LayerNorm):
                 and not (torch.is_grad_enabled() and (x.requires_grad or (w is not None and w.requires_grad))))
         if not fast:
             return super().forward(x if residual is None else x + residual)
-        import ctypes
         out = torch.empty_like(x)
-        ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
-        _lib.check(_lib.lib().mc_layernorm_f16(x.numel() // x.shape[-1], x.shape[-1], ptr(x), ptr(residual), ptr(w),
-                                                ptr(b), float(self.eps), ptr(out),
-                                                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
-                   "mc_layernorm_f16")
+        # plain integer addresses: the bound argument types are void pointers
+        st = _lib.lib().mc_layernorm_f16(x.numel() // x.shape[-1], x.shape[-1], x.data_ptr(),
+                                         None if residual is None else residual.data_ptr(),
+                                         None if w is None else w.data_ptr(), None if b is None else b.data_ptr(),
+                                         float(self.eps), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        if st:
+            _lib.check(st, "mc_layernorm_f16")
         return out
 
 
