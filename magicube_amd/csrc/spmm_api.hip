@@ -15,20 +15,19 @@ int default_config(int rhs_bits, int n, long long m_vec) {
     // MI355X over M = K = 4096, N in {128, 256, 512}, V in {2, 8}, sparsity
     // 0.7-0.98, every pair, and the reference's 2304-deep shapes: within 9 %
     // of the best tile everywhere (the autotuner covers the rest).
-    // With eight or more column tiles, eight waves share a row (one
-    // workgroup per row at n = 512 for 8-bit B) from 4096 waves on, and keep
-    // two k-blocks in flight from 8192 on.  Measured on MI355X, V = 8,
-    // s = 0.9, in one process: M = K = 4096, N = 512 (4096 waves) 8.59 us
-    // against 8.78 us for four waves per row; N = 1024 and M = 8192, N = 512
-    // (8192 waves) 14.19 and 14.15 us against 15.21 and 14.67 us; the
-    // reference's 2048 x 2304 x 512 shape (2048 waves) keeps four (4.58 us
-    // against 4.69 us for eight).
+    // With eight or more column tiles and 4096 to 8191 waves, eight waves
+    // share a row (one workgroup per row at n = 512 for 8-bit B).  Measured
+    // on MI355X over M = K = 4096, N = 512, V in {2, 4, 8}, s in {0.7, 0.9,
+    // 0.98} for L8-R8 and L16-R8: at V = 8 (4096 waves) within 1 % of the
+    // best tile, 1-2 % better than four waves per row at s = 0.7 and 0.98;
+    // at V = 2 and 4 (8192 waves and more) four waves per row are better
+    // (3-6 % at s = 0.9), and two k-blocks in flight, best there at s <= 0.9,
+    // lose 20-30 % at s = 0.98, where the rows are short.
     const int tile_cols = 16 * (32 / rhs_bits);
     const int tiles = (n + tile_cols - 1) / tile_cols;
     if (tiles >= 8) {
         const long long waves8 = m_vec * ((tiles + 7) / 8) * 8;
-        if (waves8 >= 8192) return 14;
-        if (waves8 >= 4096) return 13;
+        if (waves8 >= 4096 && waves8 < 8192) return 13;
     }
     const int tpb = tiles >= 4 ? 4 : (tiles >= 2 ? 2 : 1);
     const long long waves = m_vec * ((tiles + tpb - 1) / tpb) * tpb;
