@@ -347,15 +347,25 @@ __device__ __forceinline__ int const_load(const int *p) {
 // DBG instantiations also write the optional int32 S, int8 P and int32 P V
 // outputs; the production ones carry no stores inside the chunk loops (a
 // conditional store there would make every load wait drain the queue).
-// Four waves per SIMD: the register budget is capped at 128 (a few
-// spills), 7 % faster than three waves with the uncapped allocation.
+// Pass 2 takes the K operands of a row's first KSTASH chunks from an LDS
+// copy made in pass 1 instead of gathering them from L2 again: 12 KB per wave,
+// 52 KB per workgroup with the P tiles, so three workgroups (three waves per
+// SIMD, at most 168 registers) fit a CU's 160 KB.  Measured on 128 matrices of
+// L ~ U[512, 4096]: 0.208 ms per step against 0.233 ms without the copy (four
+// waves per SIMD), 0.215 ms with two chunks at four waves, 0.251 ms with four
+// chunks at two waves.
+constexpr int KSTASH = 3;
+
 template <int V, int QKB, int VB, int PM, int WPB, bool DBG>
-__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) void attention_kernel(const AttnMatrix *mats, const int2 *items, int n_items,
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3))) void attention_kernel(const AttnMatrix *mats, const int2 *items, int n_items,
                                                              float p_scale, int p_mode) {
     constexpr int HOME = V < 4 ? V : 4;
     constexpr int QROW = kHeadDim * QKB / 8;          // bytes per Q/K row
     constexpr int KCB = QROW / 4;                     // bytes of a K row per lane
     __shared__ __attribute__((aligned(16))) unsigned char tiles[WPB][16 * 64];
+    // K operands of a row's first KSTASH chunks, kept from pass 1 for pass 2
+    // (per wave; lane-major, so the 16-byte accesses are conflict-free)
+    __shared__ v4i kstash[WPB][KSTASH][4][64];
     // the wave id in an SGPR keeps work-item indices, descriptors and buffer
     // resources wave-uniform (no per-lane waterfall around each buffer load)
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -490,6 +500,10 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
 #pragma unroll
         for (int j = 0; j < HC; ++j) { m[j] = -INFINITY; l[j] = 0.f; }
         auto pass1_chunk = [&](int ch, const v4i (&kop)[4]) {
+            if (ch < KSTASH) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) kstash[wave][ch][t][lane] = kop[t];
+            }
             float s[4][HC];
             int si[4][4];
             logits(ch, kop, s, si);
@@ -569,20 +583,33 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
             }
         };
         {
+            // the first KSTASH chunks' K operands come from pass 1's LDS
+            // copy (the same index loads fed both passes), the rest from L2
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            auto get_krows = [&](int ch, u32 my_off, v4i (&kop)[4]) {
+                if (ch < KSTASH) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) kop[t] = kstash[wave][ch][t][lane];
+                } else {
+                    load_krows(my_off, kop);
+                }
+            };
             v4i ka[4], kb[4];
             u32 va[16], vb[16];
             u32 ia = load_idx(0);
             u32 ib = load_idx(1);
-            load_krows(ia, ka);
+            get_krows(0, ia, ka);
             load_vrows(ia, va);
             for (int ch = 0; ch < nch_w; ch += 2) {
                 ia = load_idx(ch + 2);
-                load_krows(ib, kb);
+                get_krows(ch + 1, ib, kb);
                 load_vrows(ib, vb);
                 pass2_chunk(ch, ka, va);
                 __builtin_amdgcn_sched_barrier(0);
                 ib = load_idx(ch + 3);
-                load_krows(ia, ka);
+                get_krows(ch + 2, ia, ka);
                 load_vrows(ia, va);
                 pass2_chunk(ch + 1, kb, vb);
                 __builtin_amdgcn_sched_barrier(0);
