@@ -347,14 +347,15 @@ __device__ __forceinline__ int const_load(const int *p) {
 // DBG instantiations also write the optional int32 S, int8 P and int32 P V
 // outputs; the production ones carry no stores inside the chunk loops (a
 // conditional store there would make every load wait drain the queue).
-// Pass 2 takes the K operands of a row's first KSTASH chunks from an LDS
-// copy made in pass 1 instead of gathering them from L2 again: 12 KB per wave,
-// 52 KB per workgroup with the P tiles, so three workgroups (three waves per
-// SIMD, at most 168 registers) fit a CU's 160 KB.  Measured on 128 matrices of
-// L ~ U[512, 4096]: 0.208 ms per step against 0.233 ms without the copy (four
-// waves per SIMD), 0.215 ms with two chunks at four waves, 0.251 ms with four
-// chunks at two waves.
-constexpr int KSTASH = 3;
+// Pass 2 takes the scaled logits of a row's first chunks from an LDS copy
+// made in pass 1 instead of gathering K and recomputing them: 12 KB per wave
+// (8 floats per lane and chunk at V = 8, so six chunks; 4 and twelve at
+// V = 2 and 4), 52 KB per workgroup with the P tiles, so three workgroups
+// (three waves per SIMD, at most 168 registers) fit a CU's 160 KB.  Measured
+// on 128 matrices of L ~ U[512, 4096]: 0.199 ms per step, against 0.233 ms
+// with no copy at four waves per SIMD, 0.209 ms with a copy of three chunks
+// of K instead, 0.210 ms with 8 KB of logits at four waves.
+constexpr int kLogitStashBytes = 12288;   // per wave
 
 template <int V, int QKB, int VB, int PM, int WPB, bool DBG>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3))) void attention_kernel(const AttnMatrix *mats, const int2 *items, int n_items,
@@ -363,9 +364,13 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3))) v
     constexpr int QROW = kHeadDim * QKB / 8;          // bytes per Q/K row
     constexpr int KCB = QROW / 4;                     // bytes of a K row per lane
     __shared__ __attribute__((aligned(16))) unsigned char tiles[WPB][16 * 64];
-    // K operands of a row's first KSTASH chunks, kept from pass 1 for pass 2
-    // (per wave; lane-major, so the 16-byte accesses are conflict-free)
-    __shared__ v4i kstash[WPB][KSTASH][4][64];
+    // logits s[t][j] of a row's first SST chunks, kept from pass 1 for pass 2
+    // (per wave; SQ 16-byte slots per lane, lane-major, so the accesses are
+    // conflict-free)
+    constexpr int HC_ = V == 8 ? 2 : 1;
+    constexpr int SQ = HC_;                       // 4 * HC floats per lane
+    constexpr int SST = kLogitStashBytes / (64 * 16 * SQ);
+    __shared__ v4f sstash[WPB][SST][SQ][64];
     // the wave id in an SGPR keeps work-item indices, descriptors and buffer
     // resources wave-uniform (no per-lane waterfall around each buffer load)
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -500,13 +505,18 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3))) v
 #pragma unroll
         for (int j = 0; j < HC; ++j) { m[j] = -INFINITY; l[j] = 0.f; }
         auto pass1_chunk = [&](int ch, const v4i (&kop)[4]) {
-            if (ch < KSTASH) {
-#pragma unroll
-                for (int t = 0; t < 4; ++t) kstash[wave][ch][t][lane] = kop[t];
-            }
             float s[4][HC];
             int si[4][4];
             logits(ch, kop, s, si);
+            if (ch < SST) {
+#pragma unroll
+                for (int q = 0; q < SQ; ++q) {
+                    v4f x;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) x[e] = s[(4 * q + e) / HC][(4 * q + e) % HC];
+                    sstash[wave][ch][q][lane] = x;
+                }
+            }
             softmax_chunk_update<HC>(s, m, l);
             if (DBG && M.s_out && home) {
 #pragma unroll
@@ -548,7 +558,16 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3))) v
         auto pass2_chunk = [&](int ch, const v4i (&kop)[4], const u32 (&raw_in)[16]) {
             float s[4][HC];
             int si[4][4];
-            logits(ch, kop, s, si);
+            if (ch < SST) {
+#pragma unroll
+                for (int q = 0; q < SQ; ++q) {
+                    const v4f x = sstash[wave][ch][q][lane];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) s[(4 * q + e) / HC][(4 * q + e) % HC] = x[e];
+                }
+            } else {
+                logits(ch, kop, s, si);
+            }
             int pq[4][HC];
 #pragma unroll
             for (int t = 0; t < 4; ++t)
@@ -583,18 +602,14 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3))) v
             }
         };
         {
-            // the first KSTASH chunks' K operands come from pass 1's LDS
-            // copy (the same index loads fed both passes), the rest from L2
+            // the first SST chunks' logits come from pass 1's LDS copy (the
+            // same values pass 2 would recompute), the rest from K gathered
+            // again
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             auto get_krows = [&](int ch, u32 my_off, v4i (&kop)[4]) {
-                if (ch < KSTASH) {
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) kop[t] = kstash[wave][ch][t][lane];
-                } else {
-                    load_krows(my_off, kop);
-                }
+                if (ch >= SST) load_krows(my_off, kop);
             };
             v4i ka[4], kb[4];
             u32 va[16], vb[16];

@@ -17,6 +17,7 @@ namespace mc {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v2i __attribute__((ext_vector_type(2)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 typedef unsigned int u32;
 
 __device__ __forceinline__ v4i mfma_i8(v4i a, v4i b, v4i c) {
