@@ -507,12 +507,15 @@ class AttentionBatch:
 
 
 def _np_dtype(struct) -> np.dtype:
-    """numpy record dtype with the C struct's field offsets and size."""
+    """numpy record dtype with the C struct's field offsets and size, marked
+    as an aligned struct so that numpy keeps the padding when it promotes the
+    dtype (np.concatenate of unaligned record arrays repacks them)."""
     kinds = {ctypes.c_int: np.int32, ctypes.c_float: np.float32, ctypes.c_void_p: np.uint64,
              ctypes.c_longlong: np.int64}
     names = [n for n, _ in struct._fields_]
     return np.dtype({"names": names, "formats": [kinds[t] for _, t in struct._fields_],
-                     "offsets": [getattr(struct, n).offset for n in names], "itemsize": ctypes.sizeof(struct)})
+                     "offsets": [getattr(struct, n).offset for n in names], "itemsize": ctypes.sizeof(struct)},
+                    align=True)
 
 
 _ATTN_DT = _np_dtype(_AttnMatrixC)

@@ -308,21 +308,24 @@ class FusedLayerNorm(nn.LayerNorm):
 
     def forward(self, x, residual=None):
         w, b = self.weight, self.bias
+        operands = [t for t in (x, residual, w, b) if t is not None]
         fast = (x.is_cuda and x.dtype == torch.float16 and x.is_contiguous() and len(self.normalized_shape) == 1
                 and self.normalized_shape[0] in (256, 512, 1024) and x.shape[-1] == self.normalized_shape[0]
-                and (w is None or (w.dtype == torch.float16 and w.is_cuda))
-                and (b is None or (b.dtype == torch.float16 and b.is_cuda))
-                and (residual is None or (residual.shape == x.shape and residual.dtype == x.dtype
-                                          and residual.is_cuda and residual.is_contiguous()))
-                and not (torch.is_grad_enabled() and (x.requires_grad or (w is not None and w.requires_grad))))
+                and all(t.dtype == torch.float16 and t.device == x.device for t in operands)
+                and (residual is None or (residual.shape == x.shape and residual.is_contiguous()))
+                and (w is None or w.is_contiguous()) and (b is None or b.is_contiguous())
+                # the kernel loads x and residual four halves at a time
+                and x.data_ptr() % 8 == 0 and (residual is None or residual.data_ptr() % 8 == 0)
+                and not (torch.is_grad_enabled() and any(t.requires_grad for t in operands)))
         if not fast:
             return super().forward(x if residual is None else x + residual)
         out = torch.empty_like(x)
         # plain integer addresses: the bound argument types are void pointers
-        st = _lib.lib().mc_layernorm_f16(x.numel() // x.shape[-1], x.shape[-1], x.data_ptr(),
-                                         None if residual is None else residual.data_ptr(),
-                                         None if w is None else w.data_ptr(), None if b is None else b.data_ptr(),
-                                         float(self.eps), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        with torch.cuda.device(x.device):
+            st = _lib.lib().mc_layernorm_f16(x.numel() // x.shape[-1], x.shape[-1], x.data_ptr(),
+                                             None if residual is None else residual.data_ptr(),
+                                             None if w is None else w.data_ptr(), None if b is None else b.data_ptr(),
+                                             float(self.eps), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
         if st:
             _lib.check(st, "mc_layernorm_f16")
         return out

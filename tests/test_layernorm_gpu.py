@@ -90,3 +90,26 @@ def test_fused_layernorm_module():
     y = ln(xg, residual=r)
     y.float().sum().backward()          # torch's path: differentiable
     assert xg.grad is not None
+
+
+def test_fused_layernorm_module_fallbacks():
+    """Inputs the HIP kernel does not take run torch's own add and
+    LayerNorm, bit for bit: x not 8-byte aligned, a residual that needs a
+    gradient (parameters frozen)."""
+    torch.manual_seed(4)
+    ln = spattention.FusedLayerNorm(512).cuda().half()
+    for p in ln.parameters():
+        p.requires_grad_(False)
+    buf = torch.randn(1 + 8 * 512, device="cuda").half()
+    x = buf[1:].view(8, 512)                       # 2-byte offset
+    assert x.data_ptr() % 8 != 0
+    with torch.no_grad():
+        assert torch.equal(ln(x), F.layer_norm(x, (512,), ln.weight, ln.bias, ln.eps))
+    xa = torch.randn(8, 512, device="cuda").half()
+    r = torch.randn(8, 512, device="cuda").half().requires_grad_(True)
+    y = ln(xa, residual=r)
+    assert y.requires_grad
+    y.float().sum().backward()
+    assert r.grad is not None
+    with torch.no_grad():
+        assert torch.equal(y, F.layer_norm(xa + r, (512,), ln.weight, ln.bias, ln.eps))

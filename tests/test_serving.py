@@ -130,3 +130,19 @@ def test_xcd_item_order_covers_every_row_once_and_keeps_matrices_on_one_xcd():
             x = (pos // 4) % 8
             assert xcd_of.setdefault(e, x) == x, "a matrix spans two XCDs"
     assert attention.xcd_item_order([0, 0], [0, 0]).shape == (1, 2)
+
+
+def test_record_dtypes_keep_c_layout_under_promotion():
+    """The descriptor and job record dtypes keep the C structs' padding when
+    numpy promotes them: concatenated or stacked record arrays still have
+    the 104- and 48-byte strides the kernels read (a packed 100- or 44-byte
+    stride would hand the kernels shifted pointers)."""
+    import numpy as np
+    for dt, size in ((attention._ATTN_DT, 104), (attention._JOB_DT, 48)):
+        key = "L" if "L" in dt.names else "count"
+        a = np.zeros(6, dtype=dt)
+        a[key] = np.arange(6)
+        for joined in (np.concatenate([a[:4].reshape(-1, 2)[:, 1], a[4:]]), np.stack([a[0], a[5]])):
+            assert joined.dtype.itemsize == size
+            raw = np.frombuffer(joined.tobytes(), dtype=dt)
+            assert raw[key].tolist() == joined[key].tolist()
