@@ -299,6 +299,11 @@ typedef struct mc_attn_matrix {
     void *o16;         /* non-null: fp16 output (round to nearest) at o16 +   */
     int o16_stride;    /* row * o16_stride + column (a multiple of 4, o16     */
     int o16_pad;       /* 8-byte aligned) instead of the fp32 o               */
+    const void *q16;   /* non-null: Q as fp16 rows at q16 + row * q16_stride  */
+    int q16_stride;    /* (q16 16-byte aligned, q16_stride a multiple of 8,   */
+    float q_scale;     /* (L - 1) * q16_stride + 64 < 2^30), quantized in the */
+    int q_rounding;    /* kernel as mc_attn_quantize_batched quantizes with   */
+    int q_pad;         /* q_scale and q_rounding; q is then not read          */
 } mc_attn_matrix;
 /* Fused quantized sparse attention over many matrices in one persistent
  * launch.  items is a device array of n_items (matrix index, vector row)

@@ -50,7 +50,8 @@ class _AttnMatrixC(ctypes.Structure):
                 ("v", ctypes.c_void_p), ("alpha", ctypes.c_float), ("o_scale", ctypes.c_float),
                 ("o", ctypes.c_void_p), ("o_int", ctypes.c_void_p), ("p_out", ctypes.c_void_p),
                 ("s_out", ctypes.c_void_p), ("o16", ctypes.c_void_p), ("o16_stride", ctypes.c_int),
-                ("o16_pad", ctypes.c_int)]
+                ("o16_pad", ctypes.c_int), ("q16", ctypes.c_void_p), ("q16_stride", ctypes.c_int),
+                ("q_scale", ctypes.c_float), ("q_rounding", ctypes.c_int), ("q_pad", ctypes.c_int)]
 
 
 class _QuantJobC(ctypes.Structure):
@@ -178,14 +179,36 @@ class _Dev:
     # plain-path work buffers (allocated on first use)
     work: dict = field(default_factory=dict)
 
-    def desc_bytes(self) -> bytes:
+    def q16_row_stride(self) -> int | None:
+        """Row stride (in halves) at which the fused kernel can read Q as
+        fp16 and quantize it itself, or None: rows 16-byte aligned and the
+        whole extent addressable by the kernel's 32-bit offsets."""
+        x = self.q16
+        if x.dtype != torch.float16 or not x.is_cuda or x.data_ptr() % 16:
+            return None
+        if x.is_contiguous():
+            stride = HEAD_DIM
+        elif x.dim() == 2 and x.shape[1] == HEAD_DIM and x.stride(1) == 1:
+            stride = int(x.stride(0))
+        else:
+            return None
+        if stride % 8 or (self.L - 1) * stride + HEAD_DIM >= 1 << 30:
+            return None
+        return stride
+
+    def desc_bytes(self, q_rounding: int | None = None) -> bytes:
+        """The launch descriptor; with ``q_rounding`` (the Q/K quantize
+        rounding mode) the kernel reads Q as fp16 and quantizes it itself."""
         ptr = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
         half = self.o.dtype == torch.float16
+        fq = q_rounding is not None
         return bytes(_AttnMatrixC(self.L, self.m_vec, self.row_offsets.data_ptr(), self.col_indices.data_ptr(),
                                   self.qq.data_ptr(), self.kq.data_ptr(), self.vq.data_ptr(), self.alpha,
                                   self.o_scale, 0 if half else self.o.data_ptr(), ptr(self.o_int),
                                   ptr(self.p_out), ptr(self.s_out), self.o.data_ptr() if half else 0,
-                                  int(self.o.stride(0)) if half else 0, 0))
+                                  int(self.o.stride(0)) if half else 0, 0,
+                                  self.q16.data_ptr() if fq else 0, self.q16_row_stride() if fq else 0,
+                                  float(self.scales[0]) if fq else 0.0, int(q_rounding) if fq else 0, 0))
 
     def quant_job_bytes(self) -> tuple[bytes, bytes]:
         """(q and k jobs, v job) of the batched quantize launches."""
@@ -333,11 +356,17 @@ class AttentionBatch:
     def __init__(self, mats: list[AttentionMatrix], qk_bits: int = 8, V: int = 8, p_mode: int = P_ROW_SCALED,
                  p_scale: float = 127.0, scales: list[tuple] | None = None, debug: bool = False,
                  rounding: int | None = None, device="cuda", resident: list[_Dev] | None = None,
-                 outputs: list[torch.Tensor] | None = None, v_range4: bool = False, v_bits: int = 8):
+                 outputs: list[torch.Tensor] | None = None, v_range4: bool = False, v_bits: int = 8,
+                 fuse_q: bool = True):
         """v_bits 4: V is quantized to 4 bits and held packed ([L][32] bytes),
         as the reference's rhs_pre 4 layer holds it (Q/K must be 4-bit too).
         v_range4: 8-bit V storage with values clamped to the 4-bit range
-        [-8, 7] (the same integers, one per byte)."""
+        [-8, 7] (the same integers, one per byte).
+        fuse_q: the fused kernel reads Q as fp16 and quantizes it itself
+        (the same integers as the quantize launch; the quantize launch then
+        covers K and V only, and the quantized Q buffer is left unwritten by
+        the fused path).  Taken when every matrix's Q is a CUDA fp16 [L, 64]
+        view with 16-byte aligned rows (:meth:`_Dev.q16_row_stride`)."""
         if qk_bits not in (8, 4):
             raise ValueError("qk_bits must be 8 or 4")
         if v_bits not in (8, 4) or (v_bits == 4 and qk_bits != 4):
@@ -346,6 +375,7 @@ class AttentionBatch:
             raise ValueError("unknown p_mode")
         self.qk_bits, self.V, self.p_mode, self.p_scale = qk_bits, V, p_mode, float(p_scale)
         self.v_bits = v_bits
+        self.fuse_q_requested = bool(fuse_q)
         self.debug = bool(debug)
         # the reference truncates (quantization_kernel.cu:65-69); the row-scaled layer rounds
         self.rounding = rounding if rounding is not None else (ROUND_NEAREST if p_mode == P_ROW_SCALED else ROUND_TRUNC)
@@ -380,16 +410,27 @@ class AttentionBatch:
         order = xcd_item_order([d.m_vec for d in self.mats], [d.nnz + 16 * d.m_vec for d in self.mats])
         self.n_items = int(len(order)) if any(d.m_vec for d in self.mats) else 0
         self.items = _pinned(order).to(dev, non_blocking=True)
-        self.desc = _bytes_tensor(b"".join(d.desc_bytes() for d in self.mats), dev)
+        self.fuse_q = (self.fuse_q_requested and dev.type == "cuda"
+                       and all(d.q16_row_stride() is not None for d in self.mats))
+        q_rnd = self.rounding if self.fuse_q else None
+        self.desc = _bytes_tensor(b"".join(d.desc_bytes(q_rnd) for d in self.mats), dev)
         jobs = [d.quant_job_bytes() for d in self.mats]
         qk_raw, v_raw = b"".join(j[0] for j in jobs), b"".join(j[1] for j in jobs)
         self.qk_jobs = _bytes_tensor(qk_raw, dev)
         self.v_jobs = _bytes_tensor(v_raw, dev)
+        # the fused path's first quantize launch: the Q and K jobs, or the K
+        # jobs alone (the second 48-byte record of each pair) when the kernel
+        # quantizes Q
+        n = len(self.mats)
+        job = ctypes.sizeof(_QuantJobC)
+        first_raw = b"".join(j[0][job:] for j in jobs) if self.fuse_q else qk_raw
+        self.n_first = n if self.fuse_q else 2 * n
+        self.first_jobs = _bytes_tensor(first_raw, dev) if self.fuse_q else self.qk_jobs
         # V at the width and rounding of Q and K joins their launch (the Q
         # and K jobs, then the V jobs)
-        self.all_jobs = (_bytes_tensor(qk_raw + v_raw, dev)
+        self.all_jobs = (_bytes_tensor(first_raw + v_raw, dev)
                          if self.v_bits == self.qk_bits and self.v_rounding == self.rounding
-                         and 3 * len(self.mats) <= MAX_QUANT_JOBS else None)
+                         and self.n_first + n <= MAX_QUANT_JOBS else None)
         self.max_count = max((d.L * HEAD_DIM for d in self.mats), default=0)
         # the descriptor copies run on the current stream; launches on any
         # other stream wait for them
@@ -408,10 +449,10 @@ class AttentionBatch:
         lib, st = _lib.lib(), _stream(stream)
         n = len(self.mats)
         if self.all_jobs is not None:
-            _lib.check(lib.mc_attn_quantize_batched(_dptr(self.all_jobs), 3 * n, self.qk_bits, self.rounding,
-                                                    self.max_count, st), "mc_attn_quantize_batched")
+            _lib.check(lib.mc_attn_quantize_batched(_dptr(self.all_jobs), self.n_first + n, self.qk_bits,
+                                                    self.rounding, self.max_count, st), "mc_attn_quantize_batched")
             return
-        _lib.check(lib.mc_attn_quantize_batched(_dptr(self.qk_jobs), 2 * n, self.qk_bits, self.rounding,
+        _lib.check(lib.mc_attn_quantize_batched(_dptr(self.first_jobs), self.n_first, self.qk_bits, self.rounding,
                                                 self.max_count, st), "mc_attn_quantize_batched")
         _lib.check(lib.mc_attn_quantize_batched(_dptr(self.v_jobs), n, self.v_bits, self.v_rounding,
                                                 self.max_count, st), "mc_attn_quantize_batched")

@@ -27,6 +27,11 @@ struct AttnMatrix {
     __half *o16;
     int o16_stride;
     int o16_pad;
+    const __half *q16;
+    int q16_stride;
+    float q_scale;
+    int q_rounding;
+    int q_pad;
 };
 static_assert(sizeof(AttnMatrix) == sizeof(mc_attn_matrix), "descriptor layout");
 
@@ -406,6 +411,10 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3))) v
         M.s_out = uniform(Mv.s_out);
         M.o16 = uniform(Mv.o16);
         M.o16_stride = uniform(Mv.o16_stride);
+        M.q16 = uniform(Mv.q16);
+        M.q16_stride = uniform(Mv.q16_stride);
+        M.q_scale = uniform(Mv.q_scale);
+        M.q_rounding = uniform(Mv.q_rounding);
         const int i_row = uniform(item.y);
         const int start = uniform(const_load(M.row_offsets + 2 * i_row)),
                   end = uniform(const_load(M.row_offsets + 2 * i_row + 1));
@@ -418,12 +427,43 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3))) v
         const __amdgpu_buffer_rsrc_t rq = make_rsrc(M.q, kv_bytes);
 
         // SDDMM A operand: Q rows of this vector row (rows >= V read out of range -> 0)
-        v4i qop;
-        {
+        v4i qop = v4i{0, 0, 0, 0};
+        // fp16 Q (q16 non-null): lane (c, g) loads columns 16g .. 16g + 15
+        // of row i_row * V + c here and quantizes them once the pipeline's
+        // first index and K loads are out (quantize_q16)
+        v4i qh0, qh1;
+        if (M.q16) {
+            const __amdgpu_buffer_rsrc_t rq16 =
+                make_rsrc(M.q16, (u32)(((u32)(M.L - 1) * (u32)M.q16_stride + kHeadDim) * 2u));
+            const u32 off = c < V ? ((u32)(i_row * V + c) * (u32)M.q16_stride + (u32)(16 * g)) * 2u : 0x80000000u;
+            qh0 = buf_load_b128(rq16, off);
+            qh1 = buf_load_b128(rq16, off + 16u);
+        } else {
             const u32 off = c < V ? (u32)(i_row * V + c) * QROW + (u32)(16 * g * QKB / 8) : 0x80000000u;
             qop = load_k16<QKB>(rq, off);
             if constexpr (QKB == 4) qop = expand_nibbles(qop, true, 0u);
         }
+        // the quantize launch's expression (quantize_kernel), laid out as the
+        // quantized operand would arrive: in order for int8; for int4 in
+        // expand_nibbles' order (even, then odd columns of each 8)
+        auto quantize_q16 = [&]() {
+            const bool range4 = QKB == 4 || (M.q_rounding & 2) != 0;
+            const float lo = range4 ? -8.f : -128.f, hi = range4 ? 7.f : 127.f;
+            u32 qb[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int w = e < 8 ? qh0[e >> 1] : qh1[(e - 8) >> 1];
+                const __half2 h2 = *reinterpret_cast<const __half2 *>(&w);
+                float f = ((e & 1) ? __high2float(h2) : __low2float(h2)) * M.q_scale;
+                f = fminf(fmaxf(f, lo), hi);
+                qb[e] = (u32)((M.q_rounding & 1) == 0 ? (int)f : (int)rintf(f)) & 255u;
+            }
+            const auto pack = [&](int a, int b, int c2, int d) {
+                return (int)(qb[a] | qb[b] << 8 | qb[c2] << 16 | qb[d] << 24);
+            };
+            if constexpr (QKB == 8) qop = v4i{pack(0, 1, 2, 3), pack(4, 5, 6, 7), pack(8, 9, 10, 11), pack(12, 13, 14, 15)};
+            else qop = v4i{pack(0, 2, 4, 6), pack(1, 3, 5, 7), pack(8, 10, 12, 14), pack(9, 11, 13, 15)};
+        };
 
         // ---- software pipeline over 64-slot chunks.  Per half step for chunk
         // ch: issue the index load of chunk ch+2, the row gathers of chunk ch+1
@@ -533,6 +573,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3))) v
             u32 ia = load_idx(0);
             u32 ib = load_idx(1);
             load_krows(ia, ka);
+            if (M.q16) quantize_q16();
             for (int ch = 0; ch < nch; ch += 2) {
                 ia = load_idx(ch + 2);
                 load_krows(ib, kb);

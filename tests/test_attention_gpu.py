@@ -82,6 +82,49 @@ def test_fused_equals_plain(qk_bits, p_mode, V):
         assert np.array_equal(po.view(np.uint32), fo.view(np.uint32)), "dequantized O differs"
 
 
+@pytest.mark.parametrize("rounding", [attention.ROUND_TRUNC, attention.ROUND_NEAREST])
+@pytest.mark.parametrize("qk_bits", [8, 4])
+@pytest.mark.parametrize("V", [8, 2])
+def test_kernel_quantized_q_equals_quantize_launch(V, qk_bits, rounding):
+    """The fused kernel's own quantization of fp16 Q (``fuse_q``) gives the
+    quantize launch's integers: S, P, P V and O bit for bit with and without
+    it, also with Q read in place from the middle of a [L][192] buffer.  Q
+    rows that are not 16-byte aligned, or a row stride that is not a
+    multiple of 8 halves, keep the quantize launch."""
+    mats, host = make_mats([192, 512, 1000], V=V, sparsity=0.8, seed=70 + V + qk_bits + rounding)
+    strided, unaligned, odd = [], [], []
+    for m in mats:
+        buf = torch.zeros(m.L, 192, dtype=torch.float16, device="cuda")
+        buf[:, 64:128] = m.q
+        strided.append(attention.AttentionMatrix(buf[:, 64:128], m.k, m.v, m.mask))
+        flat = torch.zeros(m.L * 64 + 4, dtype=torch.float16, device="cuda")
+        flat[4:] = m.q.reshape(-1)
+        unaligned.append(attention.AttentionMatrix(flat[4:].view(m.L, 64), m.k, m.v, m.mask))
+        buf2 = torch.zeros(m.L, 196, dtype=torch.float16, device="cuda")
+        buf2[:, :64] = m.q
+        odd.append(attention.AttentionMatrix(buf2[:, :64], m.k, m.v, m.mask))
+    kw = dict(qk_bits=qk_bits, V=V, p_mode=attention.P_ROW_SCALED, p_scale=127.0, debug=True, rounding=rounding)
+    ref = attention.AttentionBatch(mats, fuse_q=False, **kw)
+    kw["scales"] = [d.scales for d in ref.mats]
+    batches = [attention.AttentionBatch(x, **kw) for x in (mats, strided, unaligned, odd)]
+    assert not ref.fuse_q and [b.fuse_q for b in batches] == [True, True, False, False]
+    assert batches[0].n_first == len(mats) and ref.n_first == 2 * len(mats)     # K jobs only, or Q and K
+    assert (batches[0].all_jobs is not None) == (qk_bits == 8)                    # int8 V joins int8 K
+    print(f"kernel-quantized Q: V={V} qk_bits={qk_bits} rounding={rounding}", flush=True)
+    for b in [ref] + batches:
+        b.forward()
+    torch.cuda.synchronize()
+    want = _debug_outputs(ref)
+    for b in batches:
+        for d, w, g in zip(b.mats, want, _debug_outputs(b)):
+            real = np.concatenate([np.arange(d.pairs[2 * i], d.pairs[2 * i + 1]) for i in range(d.m_vec)])
+            idx = (real[:, None] * V + np.arange(V)[None, :]).ravel()
+            assert np.array_equal(w[0][idx], g[0][idx]), "int32 logits differ"
+            assert np.array_equal(w[1][: d.aligned_num * V], g[1][: d.aligned_num * V]), "int8 P differs"
+            assert np.array_equal(w[2], g[2]), "int32 P V differs"
+            assert np.array_equal(w[3].view(np.uint32), g[3].view(np.uint32)), "dequantized O differs"
+
+
 @pytest.mark.parametrize("p_mode", [attention.P_ROW_SCALED, attention.P_REFERENCE, attention.P_REFERENCE_4BIT])
 @pytest.mark.parametrize("qk_bits", [8, 4])
 @pytest.mark.parametrize("V", [8, 4, 2])

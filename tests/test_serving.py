@@ -42,6 +42,7 @@ def test_resident_block_descriptors_match_per_matrix_path():
             if f in ("row_offsets", "col_indices"):
                 continue            # the block uploads its own copy of a host mask
             assert np.array_equal(blk.desc[f], ref[f]), f
+        assert not blk.desc["q16"].any()       # blocks launch with Q quantized by the quantize launch
         # operand regions: q, k sized L*64*bits/8, v L*64, o L*64*4, inside the arena, disjoint
         base = blk.arena.data_ptr()
         spans = []
@@ -88,9 +89,9 @@ def test_block_batch_device_bytes_decode_to_valid_operands():
     for bits in (8, 4):
         bb = attention.BlockBatch([b for w, b in blocks if w == bits], device="cpu")
         raw = lambda t: t.numpy().tobytes()  # noqa: E731
-        # 104-byte descriptors and 48-byte jobs (include/magicube_hip.h)
-        assert attention._ATTN_DT.itemsize == 104 and attention._JOB_DT.itemsize == 48
-        assert bb.desc.numel() == 104 * bb.n_mats and bb.qk_jobs.numel() == 2 * 48 * bb.n_mats
+        # 128-byte descriptors and 48-byte jobs (include/magicube_hip.h)
+        assert attention._ATTN_DT.itemsize == 128 and attention._JOB_DT.itemsize == 48
+        assert bb.desc.numel() == 128 * bb.n_mats and bb.qk_jobs.numel() == 2 * 48 * bb.n_mats
         assert bb.v_jobs.numel() == 48 * bb.n_mats
         desc = np.frombuffer(raw(bb.desc), dtype=attention._ATTN_DT)
         for j in np.frombuffer(raw(bb.qk_jobs), dtype=attention._JOB_DT).tolist():
@@ -135,10 +136,11 @@ def test_xcd_item_order_covers_every_row_once_and_keeps_matrices_on_one_xcd():
 def test_record_dtypes_keep_c_layout_under_promotion():
     """The descriptor and job record dtypes keep the C structs' padding when
     numpy promotes them: concatenated or stacked record arrays still have
-    the 104- and 48-byte strides the kernels read (a packed 100- or 44-byte
-    stride would hand the kernels shifted pointers)."""
+    the 128- and 48-byte strides the kernels read (the job struct has 4
+    bytes of padding after its scale: a packed 44-byte stride would hand the
+    kernel shifted pointers)."""
     import numpy as np
-    for dt, size in ((attention._ATTN_DT, 104), (attention._JOB_DT, 48)):
+    for dt, size in ((attention._ATTN_DT, 128), (attention._JOB_DT, 48)):
         key = "L" if "L" in dt.names else "count"
         a = np.zeros(6, dtype=dt)
         a[key] = np.arange(6)
