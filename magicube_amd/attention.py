@@ -7,8 +7,9 @@ HBM: fp16 inputs, quantized operands, masks, outputs and the device
 descriptors of the fused launch.  It runs
 
 * :meth:`AttentionBatch.forward` — one batched quantize launch per operand
-  width plus ONE persistent fused launch over a longest-first work list of
-  (matrix, vector row) items (capturable in a hipGraph), or
+  width (K and V; Q too when the fused kernel does not read it as fp16) plus
+  ONE persistent fused launch over a longest-first work list of (matrix,
+  vector row) items (capturable in a hipGraph), or
 * :meth:`AttentionBatch.forward_plain` — the one-op-per-launch pipeline per
   matrix (quantize x3, SDDMM, softmax + requantize, SpMM, dequantize).
 

@@ -276,7 +276,7 @@ class SpMultiheadAttention(nn.MultiheadAttention):
             self._plans.move_to_end(key_)
 
         # in-projection into the plan's [L][batch][3E] buffer, read in place
-        # by the quantize launch
+        # by the quantize launch (K, V) and the fused kernel (Q)
         if query.dtype != torch.float16:
             raise NotImplementedError("the quantized path takes fp16 activations")
         w, b = self.in_proj_weight, self.in_proj_bias
