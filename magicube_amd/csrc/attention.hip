@@ -65,8 +65,10 @@ __global__ __launch_bounds__(256) void quantize_kernel(const QuantJob *jobs, int
                                        : jb.x + base;
         float x[8];
         if (base + 8 <= jb.count && (reinterpret_cast<uintptr_t>(src) & 15u) == 0) {
-            // one 16-byte load for the 8 values
-            const v4i w = *reinterpret_cast<const v4i *>(src);
+            // one 16-byte load for the 8 values, non-temporal: the fp16 source is
+            // read once, and keeping it out of the caches leaves them to the
+            // quantized operands the fused kernel gathers next
+            const v4i w = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(src));
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int we = w[e];
