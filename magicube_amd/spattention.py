@@ -422,8 +422,11 @@ class _Encoder(nn.Module):
         self.linear = nn.Linear(embed_dim, num_class)
 
     def forward(self, x):
-        out = self.encoder(x) * math.sqrt(self.embed_dim)
-        out = torch.transpose(out, 0, 1)
+        # the embedding of the transposed tokens: the reference's
+        # transpose(embedding(x), 0, 1) value for value, but laid out
+        # contiguously as [L][batch][E], so that the first block's LayerNorms
+        # take the fused kernel
+        out = self.encoder(x.t()) * math.sqrt(self.embed_dim)
         out = self.pos_encoder(out)
         for blk in self.blocks:
             out = blk(out)

@@ -136,3 +136,28 @@ def test_spmm_and_sddmm_mirrors_reject_short_buffers():
     with pytest.raises(ValueError, match="lhs_matrix"):
         sddmm.wmma_sddmm(8, 4, 64, 64, z, z, z, torch.zeros(4 * 8 * 64 // 4 - 1, dtype=torch.int32),
                          torch.zeros(64 * 64 // 4, dtype=torch.int32), z, 8)
+
+
+def test_encoder_embeds_transposed_tokens_contiguously():
+    """The encoder's [L][batch][E] input is embedding(x.t()) * sqrt(E): the
+    reference's transpose(embedding(x) * sqrt(E), 0, 1) bit for bit
+    (end_to_end.py:160-166), laid out contiguously so the first block's
+    LayerNorms take the fused kernel; the dense encoder's forward equals the
+    transposed formulation's."""
+    import math
+    import torch
+    torch.manual_seed(0)
+    model = spattention.TransformerEncoder(vocab_size=37, embed_dim=64, num_heads=1, seq_len=32, mlp_dim=128,
+                                           num_class=5, layers=1).eval()
+    x = torch.randint(0, 37, (3, 32))
+    ours = model.encoder(x.t()) * math.sqrt(64)
+    ref = torch.transpose(model.encoder(x) * math.sqrt(64), 0, 1)
+    assert ours.is_contiguous() and not ref.is_contiguous()
+    assert torch.equal(ours, ref)
+    with torch.no_grad():
+        got = model(x)
+        out = model.pos_encoder(ref)
+        for blk in model.blocks:
+            out = blk(out)
+        want = model.linear(model.layer_norm(out))
+    assert torch.allclose(got, want, rtol=0, atol=1e-6)
