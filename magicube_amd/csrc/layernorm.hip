@@ -18,9 +18,24 @@
 namespace mc {
 namespace {
 
+// x + the value of the lane a DPP pattern names (within each row of 16 lanes)
+#define LN_DPP_ADD(x, ctrl) \
+    ((x) + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), (ctrl), 0xF, 0xF, false)))
+
+// Sum over the wave, the same value in every lane: within each row of 16
+// lanes by DPP (lane pairs 1 apart, 2 apart, then the half-row and row
+// mirrors; each step adds the same two values in both lanes of a pair),
+// then the four row sums read with v_readlane.  No LDS traffic.
 __device__ __forceinline__ float ln_wave_sum(float x) {
-    for (int s = 1; s < 64; s <<= 1) x += __shfl_xor(x, s, 64);
-    return x;
+    x = LN_DPP_ADD(x, 0xB1);     // quad_perm [1, 0, 3, 2]
+    x = LN_DPP_ADD(x, 0x4E);     // quad_perm [2, 3, 0, 1]
+    x = LN_DPP_ADD(x, 0x141);    // row_half_mirror
+    x = LN_DPP_ADD(x, 0x140);    // row_mirror
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 48));
+    return (r0 + r1) + (r2 + r3);
 }
 
 // Rows per wave: a wave issues every load of its rows before the first
