@@ -521,7 +521,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3))) v
         auto load_vrows = [&](u32 my_off, u32 (&raw)[16]) {
 #pragma unroll
             for (int p = 0; p < 16; ++p) {
-                u32 row = (u32)__shfl((int)my_off, 16 * g + p, 64);
+                u32 row = (u32)row_bcast16((int)my_off, p);   // lane 16g + p
                 if constexpr (VB == 4) {
                     row = (row >> 1) | (row & kRowOob);   // 32-byte rows, marker kept
                     raw[p] = buf_load_u16(rv, row + 2u * c);

@@ -123,6 +123,28 @@ __device__ __forceinline__ int lane_down(int x, int delta) {
     const int lane = __lane_id();
     return __builtin_amdgcn_ds_bpermute(((lane + delta) & 63) << 2, x);
 }
+// Lane p of each row of 16 lanes, broadcast to that row (DPP row_newbcast,
+// no LDS traffic); p must fold to a constant.
+__device__ __forceinline__ int row_bcast16(int x, int p) {
+    switch (p & 15) {
+    case 0: return __builtin_amdgcn_update_dpp(0, x, 0x150, 0xF, 0xF, false);
+    case 1: return __builtin_amdgcn_update_dpp(0, x, 0x151, 0xF, 0xF, false);
+    case 2: return __builtin_amdgcn_update_dpp(0, x, 0x152, 0xF, 0xF, false);
+    case 3: return __builtin_amdgcn_update_dpp(0, x, 0x153, 0xF, 0xF, false);
+    case 4: return __builtin_amdgcn_update_dpp(0, x, 0x154, 0xF, 0xF, false);
+    case 5: return __builtin_amdgcn_update_dpp(0, x, 0x155, 0xF, 0xF, false);
+    case 6: return __builtin_amdgcn_update_dpp(0, x, 0x156, 0xF, 0xF, false);
+    case 7: return __builtin_amdgcn_update_dpp(0, x, 0x157, 0xF, 0xF, false);
+    case 8: return __builtin_amdgcn_update_dpp(0, x, 0x158, 0xF, 0xF, false);
+    case 9: return __builtin_amdgcn_update_dpp(0, x, 0x159, 0xF, 0xF, false);
+    case 10: return __builtin_amdgcn_update_dpp(0, x, 0x15A, 0xF, 0xF, false);
+    case 11: return __builtin_amdgcn_update_dpp(0, x, 0x15B, 0xF, 0xF, false);
+    case 12: return __builtin_amdgcn_update_dpp(0, x, 0x15C, 0xF, 0xF, false);
+    case 13: return __builtin_amdgcn_update_dpp(0, x, 0x15D, 0xF, 0xF, false);
+    case 14: return __builtin_amdgcn_update_dpp(0, x, 0x15E, 0xF, 0xF, false);
+    default: return __builtin_amdgcn_update_dpp(0, x, 0x15F, 0xF, 0xF, false);
+    }
+}
 __device__ __forceinline__ int lane_read(int x, int src_lane) {
     return __builtin_amdgcn_ds_bpermute((src_lane & 63) << 2, x);
 }
